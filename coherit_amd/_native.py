@@ -1,0 +1,138 @@
+"""ctypes binding of ``libcoherit_amd.so`` (the C-ABI in include/coherit_amd.h).
+
+The product path has no CPU fallback: if the library is missing or no GPU is
+visible, :func:`lib` raises.  Status codes map to the exception types the
+reference raises for the same conditions.
+"""
+from __future__ import annotations
+
+import ctypes
+import pathlib
+import threading
+
+_LIB_PATH = pathlib.Path(__file__).resolve().parent / "_lib" / "libcoherit_amd.so"
+
+COH_OK = 0
+COH_ERR_VALUE = 1
+COH_ERR_FLOAT = 2
+COH_ERR_RUNTIME = 3
+COH_ERR_HIP = 4
+COH_ERR_INDEX = 5
+
+PREC_FP32 = 0
+PREC_I8X3 = 1
+PREC_I8X2 = 2
+PRECISIONS = {"fp32": PREC_FP32, "i8x3": PREC_I8X3, "i8x2": PREC_I8X2}
+
+OP_KV = 0
+OP_XTV = 1
+
+_EXC = {
+    COH_ERR_VALUE: ValueError,
+    COH_ERR_FLOAT: FloatingPointError,
+    COH_ERR_RUNTIME: RuntimeError,
+    COH_ERR_HIP: RuntimeError,
+    COH_ERR_INDEX: IndexError,
+}
+
+_i64 = ctypes.c_int64
+_i32 = ctypes.c_int32
+_u32 = ctypes.c_uint32
+_u64 = ctypes.c_uint64
+_f32 = ctypes.c_float
+_f64 = ctypes.c_double
+_p = ctypes.c_void_p
+_pp = ctypes.POINTER(ctypes.c_void_p)
+
+# name -> argtypes (all functions return int status unless listed in _RESTYPE)
+SIGNATURES = {
+    "coh_last_error": [],
+    "coh_version": [],
+    "coh_panel_create": [_i64, _i32, _p, _p, _p, _i32, _pp],
+    "coh_panel_destroy": [_p],
+    "coh_panel_info": [_p, _p],
+    "coh_panel_component_slots": [_p, _p, _p],
+    "coh_panel_call_slots": [_p, _p],
+    "coh_panel_ingest_packed_call": [_p, _i32, _p, _i64, _p],
+    "coh_panel_ingest_i8_varmaj": [_p, _i64, _i64, _p, _i64, _i32, _p],
+    "coh_panel_ingest_bed": [_p, _i64, _i64, _p, _i64, _p, _p],
+    "coh_panel_generate": [_p, _u64, _p, _p, _p, _u32, _p],
+    "coh_panel_finalize": [_p, _p, _p, _p],
+    "coh_panel_copy_stats": [_p, _p, _p, _p, _p, _p, _p],
+    "coh_panel_read_packed_call": [_p, _i32, _p, _i64, _p],
+    "coh_workspace_bytes": [_p, _i32, _i32, _i64, _i32, _i32, _p],
+    "coh_kv": [_p, _i32, _p, _i64, _i64, _i32, _i32, _p, _i32, _p, _i64, _i64, _f32, _p, _u64, _p],
+    "coh_xtv": [_p, _i32, _p, _i64, _i64, _i32, _i32, _f32, _i32, _p, _i64, _p, _u64, _p],
+    "coh_zb": [_p, _i32, _p, _i64, _i64, _i32, _i32, _i32, _p, _i64, _i64, _f32, _p, _u64, _p],
+    "coh_exact_diag": [_p, _f64, _p, _p],
+    "coh_reduce_workspace_bytes": [_i64, _i64, _p],
+    "coh_col_dots": [_i64, _i64, _p, _i64, _p, _i64, _p, _p, _u64, _p],
+    "coh_pcg_state_update": [_i64, _i64, _p, _i64, _p, _i64, _p, _i64, _p, _i64, _p, _p, _p, _p,
+                             _u64, _p],
+    "coh_pcg_direction_update": [_i64, _i64, _p, _i64, _p, _i64, _p, _i64, _p, _p, _p, _p, _p,
+                                 _u64, _p],
+    "coh_lanczos_step": [_i64, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _u64, _p],
+}
+_RESTYPE = {"coh_last_error": ctypes.c_char_p}
+
+_lock = threading.Lock()
+_lib = None
+
+
+def load_library(path: pathlib.Path | str | None = None):
+    """Load the shared library without touching the GPU (symbol checks)."""
+    p = pathlib.Path(path) if path is not None else _LIB_PATH
+    if not p.exists():
+        raise RuntimeError(
+            f"coherit_amd HIP library not built: {p} is missing "
+            "(run `python -m coherit_amd.build` or __graft_entry__.build())."
+        )
+    dll = ctypes.CDLL(str(p))
+    for name, args in SIGNATURES.items():
+        fn = getattr(dll, name)
+        fn.argtypes = args
+        fn.restype = _RESTYPE.get(name, ctypes.c_int)
+    return dll
+
+
+def lib():
+    """The loaded library; raises when the library or a GPU is missing."""
+    global _lib
+    if _lib is None:
+        with _lock:
+            if _lib is None:
+                import torch
+
+                if not torch.cuda.is_available():
+                    raise RuntimeError(
+                        "coherit_amd requires a ROCm GPU (gfx950); none is visible. "
+                        "There is no CPU fallback."
+                    )
+                _lib = load_library()
+    return _lib
+
+
+def check(status: int) -> None:
+    if status != COH_OK:
+        msg = _lib.coh_last_error().decode() if _lib is not None else "unknown error"
+        raise _EXC.get(status, RuntimeError)(msg)
+
+
+def call(name: str, *args) -> None:
+    check(getattr(lib(), name)(*args))
+
+
+def ptr(t) -> int | None:
+    """Device (or host) address of a torch tensor / numpy array, None for None."""
+    if t is None:
+        return None
+    if hasattr(t, "data_ptr"):
+        return t.data_ptr()
+    return t.ctypes.data
+
+
+def stream_ptr(stream=None) -> int:
+    import torch
+
+    s = torch.cuda.current_stream() if stream is None else stream
+    return s.cuda_stream

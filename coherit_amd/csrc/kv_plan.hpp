@@ -1,0 +1,99 @@
+// Launch planning shared by the operator kernels.
+#pragma once
+
+#include "common.hpp"
+
+namespace coh {
+
+constexpr int kMaxComp = 64;          // components per operator call
+constexpr int64_t kTargetBlocks = 2048;  // ~8 workgroups per CU on 256 CUs
+
+// Per-component scale factors (passed by value to kernels).
+struct CompScale {
+  int ncomp;
+  int64_t end[kMaxComp];
+  float scale[kMaxComp];
+  __device__ float scale_of(int64_t slot) const {
+    for (int g = 0; g < ncomp; ++g)
+      if (slot < end[g]) return scale[g];
+    return 0.0f;
+  }
+};
+
+// Slot chunks of the phase-2 contraction; a chunk never straddles components.
+struct ChunkTable {
+  int ncomp;
+  int nchunks;
+  int64_t chunk_len;
+  int64_t begin[kMaxComp];
+  int64_t end[kMaxComp];
+  int chunk_off[kMaxComp + 1];
+  __device__ void chunk_range(int y, int64_t* j0, int64_t* j1) const {
+    int g = 0;
+    while (g + 1 < ncomp && y >= chunk_off[g + 1]) ++g;
+    const int64_t a = begin[g] + (int64_t)(y - chunk_off[g]) * chunk_len;
+    *j0 = a;
+    *j1 = (a + chunk_len < end[g]) ? a + chunk_len : end[g];
+  }
+};
+
+struct KvPlan {
+  int precision;
+  int64_t slot0, slot1;     // slot range of the requested components
+  int KC;                   // VALU column chunk
+  int nsplit1;              // phase-1 sample splits
+  int64_t rows_per_split;
+  int64_t chunk_len;
+  ChunkTable tab;
+  int64_t ldu;
+  // workspace offsets (bytes)
+  int64_t off_U, off_p1, off_p2, off_q, off_aux, total;
+  // MFMA plan fields
+  int limbs;
+  int64_t n_pad;            // samples padded to the MFMA K tile
+};
+
+void build_chunks(const coh_panel* p, int32_t cb, int32_t ce, int64_t chunk_len, ChunkTable* t);
+int plan_valu(const coh_panel* p, int64_t k, int32_t cb, int32_t ce, KvPlan* pl);
+int run_phase1_valu(const coh_panel* p, const KvPlan& pl, const float* V, int64_t k, int64_t ldv,
+                    const CompScale& cs, int inv_pow, uint8_t* ws, hipStream_t st);
+int run_phase2_valu(const coh_panel* p, const KvPlan& pl, int64_t k, int stacked, float* out,
+                    int64_t ldo, int64_t out_stride, float beta, uint8_t* ws, hipStream_t st);
+
+// MFMA int8-limb path (kv_mfma.hip)
+int plan_mfma(const coh_panel* p, int precision, int64_t k, int32_t cb, int32_t ce, KvPlan* pl);
+int run_phase1_mfma(const coh_panel* p, const KvPlan& pl, const float* V, int64_t k, int64_t ldv,
+                    const CompScale& cs, int inv_pow, uint8_t* ws, hipStream_t st);
+int run_phase2_mfma(const coh_panel* p, const KvPlan& pl, int64_t k, int stacked, float* out,
+                    int64_t ldo, int64_t out_stride, float beta, uint8_t* ws, hipStream_t st);
+
+inline int make_plan(const coh_panel* p, int precision, int64_t k, int32_t cb, int32_t ce,
+                     KvPlan* pl) {
+  pl->precision = precision;
+  if (precision == COH_PREC_FP32) return plan_valu(p, k, cb, ce, pl);
+  if (precision == COH_PREC_I8X3 || precision == COH_PREC_I8X2)
+    return plan_mfma(p, precision, k, cb, ce, pl);
+  return fail(COH_ERR_VALUE, "unknown precision %d", precision);
+}
+
+inline int run_xtv(const coh_panel* p, const KvPlan& pl, int precision, const float* V,
+                   int64_t k, int64_t ldv, const CompScale& cs, uint8_t* ws, hipStream_t st) {
+  if (precision == COH_PREC_FP32) return run_phase1_valu(p, pl, V, k, ldv, cs, 1, ws, st);
+  return run_phase1_mfma(p, pl, V, k, ldv, cs, 1, ws, st);
+}
+
+inline int run_kv(const coh_panel* p, const KvPlan& pl, int precision, const float* V, int64_t k,
+                  int64_t ldv, const CompScale& cs, int stacked, float* out, int64_t ldo,
+                  int64_t out_stride, float beta, uint8_t* ws, hipStream_t st) {
+  int s;
+  if (precision == COH_PREC_FP32) {
+    s = run_phase1_valu(p, pl, V, k, ldv, cs, 2, ws, st);
+    if (s != COH_OK) return s;
+    return run_phase2_valu(p, pl, k, stacked, out, ldo, out_stride, beta, ws, st);
+  }
+  s = run_phase1_mfma(p, pl, V, k, ldv, cs, 2, ws, st);
+  if (s != COH_OK) return s;
+  return run_phase2_mfma(p, pl, k, stacked, out, ldo, out_stride, beta, ws, st);
+}
+
+}  // namespace coh

@@ -1,0 +1,285 @@
+// Fused column-reduction and update kernels for block PCG (pcg.py:31-65)
+// and the affine Lanczos recurrence (reml.py:284-304).  Vectors are (n, k)
+// row-major FP32; per-column scalars are accumulated and kept in FP64.
+// Reductions are two-stage with a fixed summation order (deterministic).
+#include "common.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace coh {
+
+constexpr int kMaxRedBlocks = 1024;
+
+struct RedPlan {
+  int nblk;
+  int ctiles;
+};
+
+static RedPlan red_plan(int64_t n, int64_t k) {
+  RedPlan r;
+  const int64_t kk = std::min<int64_t>(k, 256);
+  r.nblk = (int)std::max<int64_t>(1, std::min<int64_t>(kMaxRedBlocks, ceil_div(n * kk, 4096)));
+  r.ctiles = (int)ceil_div(k, 256);
+  return r;
+}
+
+static uint64_t red_ws_bytes(int64_t k) {
+  // partials + 4 scratch vectors of k doubles
+  return (uint64_t)round_up((int64_t)kMaxRedBlocks * k * 8, 256) + (uint64_t)round_up(4 * k * 8, 256);
+}
+
+// part[blk][c] = sum over this block's rows of A[i,c] * (B[i,c] - s[c]*C[i,c])
+__global__ __launch_bounds__(256) void k_coldot_part(int64_t n, int64_t k, const float* __restrict__ A,
+                                                     int64_t lda, const float* __restrict__ B,
+                                                     int64_t ldb, const float* __restrict__ C,
+                                                     int64_t ldc, const double* __restrict__ s,
+                                                     double* __restrict__ part) {
+  const int64_t c0 = (int64_t)blockIdx.y * 256;
+  const int kk = (int)min<int64_t>(256, k - c0);
+  const int R = 256 / kk;
+  const int t = threadIdx.x;
+  const int c = t % kk, r = t / kk;
+  double acc = 0.0;
+  if (r < R) {
+    const int64_t col = c0 + c;
+    const float sc = (C != nullptr) ? static_cast<float>(s[col]) : 0.0f;
+    for (int64_t i = (int64_t)blockIdx.x * R + r; i < n; i += (int64_t)gridDim.x * R) {
+      float b = B[i * ldb + col];
+      if (C != nullptr) b = b - sc * C[i * ldc + col];
+      acc += static_cast<double>(A[i * lda + col]) * static_cast<double>(b);
+    }
+  }
+  __shared__ double sh[256];
+  sh[t] = acc;
+  __syncthreads();
+  if (t < kk) {
+    double sum = 0.0;
+    for (int rr = 0; rr < R; ++rr) sum += sh[rr * kk + t];
+    part[(int64_t)blockIdx.x * k + c0 + t] = sum;
+  }
+}
+
+__global__ void k_coldot_final(const double* __restrict__ part, int nblk, int64_t k,
+                               double* __restrict__ out) {
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < k;
+       c += (int64_t)gridDim.x * blockDim.x) {
+    double sum = 0.0;
+    for (int b = 0; b < nblk; ++b) sum += part[(int64_t)b * k + c];
+    out[c] = sum;
+  }
+}
+
+static int coldot(int64_t n, int64_t k, const float* A, int64_t lda, const float* B, int64_t ldb,
+                  const float* C, int64_t ldc, const double* s, double* out, uint8_t* ws,
+                  hipStream_t st) {
+  const RedPlan rp = red_plan(n, k);
+  double* part = reinterpret_cast<double*>(ws);
+  hipLaunchKernelGGL(k_coldot_part, dim3(rp.nblk, rp.ctiles), dim3(256), 0, st, n, k, A, lda, B,
+                     ldb, C, ldc, s, part);
+  COH_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_coldot_final, dim3((unsigned)ceil_div(k, 256)), dim3(256), 0, st, part,
+                     rp.nblk, k, out);
+  COH_CHECK_LAUNCH();
+  return COH_OK;
+}
+
+__device__ inline bool finite_d(double x) { return isfinite(x); }
+
+// alpha per column (pcg.py:39-44) and the breakdown flag.
+__global__ void k_pcg_alpha(int64_t k, const double* __restrict__ rs, const double* __restrict__ denom,
+                            const int32_t* __restrict__ active, double* __restrict__ alpha,
+                            int32_t* __restrict__ breakdown) {
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < k;
+       c += (int64_t)gridDim.x * blockDim.x) {
+    const double r = rs[c], d = denom[c];
+    const bool valid = (r > 0.0) && (d > 0.0) && finite_d(r) && finite_d(d);
+    const bool moving = active[c] && valid;
+    alpha[c] = moving ? r / d : 0.0;
+    if (active[c] && !valid) atomicOr(breakdown, 1);
+  }
+}
+
+__global__ void k_pcg_xr(int64_t n, int64_t k, float* __restrict__ X, int64_t ldx,
+                         float* __restrict__ R, int64_t ldr, const float* __restrict__ P,
+                         int64_t ldp, const float* __restrict__ AP, int64_t ldap,
+                         const double* __restrict__ alpha) {
+  const int64_t total = n * k;
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = idx / k;
+    const int64_t c = idx % k;
+    const float a = static_cast<float>(alpha[c]);
+    X[i * ldx + c] = X[i * ldx + c] + P[i * ldp + c] * a;
+    R[i * ldr + c] = R[i * ldr + c] - AP[i * ldap + c] * a;
+  }
+}
+
+// pcg.py:58-65 scalar part.
+__global__ void k_pcg_dir_scalars(int64_t k, const double* __restrict__ rs_new,
+                                  const double* __restrict__ rr, double* __restrict__ rs,
+                                  const double* __restrict__ threshold, double* __restrict__ rnorm,
+                                  int32_t* __restrict__ active, double* __restrict__ beta) {
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < k;
+       c += (int64_t)gridDim.x * blockDim.x) {
+    const double rn = sqrt(rr[c]);
+    const bool act = rn > threshold[c];
+    const double old = rs[c];
+    beta[c] = act ? rs_new[c] / (old > 0.0 ? old : 1.0) : 0.0;
+    rs[c] = act ? rs_new[c] : 0.0;
+    rnorm[c] = rn;
+    active[c] = act ? 1 : 0;
+  }
+}
+
+__global__ void k_pcg_p(int64_t n, int64_t k, float* __restrict__ P, int64_t ldp,
+                        const float* __restrict__ Z, int64_t ldz, const int32_t* __restrict__ active,
+                        const double* __restrict__ beta) {
+  const int64_t total = n * k;
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = idx / k;
+    const int64_t c = idx % k;
+    P[i * ldp + c] = active[c] ? Z[i * ldz + c] + P[i * ldp + c] * static_cast<float>(beta[c]) : 0.0f;
+  }
+}
+
+// w = Kv - beta_prev*prev - alpha*v (reml.py:298-300), written to out.
+__global__ void k_lanczos_w(int64_t n, int64_t k, const float* __restrict__ v,
+                            const float* __restrict__ Kv, const float* prev,
+                            const double* __restrict__ beta_prev, const double* __restrict__ alpha,
+                            float* out) {
+  const int64_t total = n * k;
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t c = idx % k;
+    const float w = Kv[idx] - prev[idx] * static_cast<float>(beta_prev[c]);
+    out[idx] = w - v[idx] * static_cast<float>(alpha[c]);
+  }
+}
+
+__global__ void k_sqrt_vec(int64_t k, const double* __restrict__ ww, double* __restrict__ beta) {
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < k;
+       c += (int64_t)gridDim.x * blockDim.x)
+    beta[c] = sqrt(ww[c]);
+}
+
+__global__ void k_safe_div(int64_t n, int64_t k, float* __restrict__ x,
+                           const double* __restrict__ beta) {
+  const int64_t total = n * k;
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const double b = beta[idx % k];
+    x[idx] = x[idx] / static_cast<float>(b > 0.0 ? b : 1.0);
+  }
+}
+
+static unsigned ew_grid(int64_t total) {
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(total, 256), 16384));
+}
+
+}  // namespace coh
+
+using namespace coh;
+
+extern "C" int coh_reduce_workspace_bytes(int64_t n, int64_t k, uint64_t* bytes) {
+  COH_REQUIRE(bytes != nullptr, COH_ERR_VALUE, "NULL argument");
+  COH_REQUIRE(n >= 0 && k >= 1, COH_ERR_VALUE, "invalid shape (n=%lld, k=%lld)", (long long)n,
+              (long long)k);
+  *bytes = red_ws_bytes(k);
+  return COH_OK;
+}
+
+#define COH_WS_CHECK(ws, bytes, k)                                                         \
+  COH_REQUIRE((ws) != nullptr && (bytes) >= red_ws_bytes(k), COH_ERR_VALUE,                \
+              "reduction workspace too small")
+
+extern "C" int coh_col_dots(int64_t n, int64_t k, const float* A, int64_t lda, const float* B,
+                            int64_t ldb, double* out, void* ws, uint64_t ws_bytes, void* stream) {
+  COH_REQUIRE(A && B && out, COH_ERR_VALUE, "NULL argument");
+  COH_REQUIRE(n >= 0 && k >= 1 && lda >= k && ldb >= k, COH_ERR_VALUE, "invalid shape");
+  COH_WS_CHECK(ws, ws_bytes, k);
+  return coldot(n, k, A, lda, B, ldb, nullptr, 0, nullptr, out, static_cast<uint8_t*>(ws),
+                (hipStream_t)stream);
+}
+
+extern "C" int coh_pcg_state_update(int64_t n, int64_t k, float* X, int64_t ldx, float* R,
+                                    int64_t ldr, const float* P, int64_t ldp, const float* AP,
+                                    int64_t ldap, const double* rs, const int32_t* active,
+                                    int32_t* breakdown, void* ws, uint64_t ws_bytes,
+                                    void* stream) {
+  COH_REQUIRE(X && R && P && AP && rs && active && breakdown, COH_ERR_VALUE, "NULL argument");
+  COH_REQUIRE(n >= 0 && k >= 1 && ldx >= k && ldr >= k && ldp >= k && ldap >= k, COH_ERR_VALUE,
+              "invalid shape");
+  COH_WS_CHECK(ws, ws_bytes, k);
+  hipStream_t st = (hipStream_t)stream;
+  uint8_t* w = static_cast<uint8_t*>(ws);
+  double* scratch = reinterpret_cast<double*>(w + round_up((int64_t)kMaxRedBlocks * k * 8, 256));
+  double* denom = scratch;
+  double* alpha = scratch + k;
+  int s = coldot(n, k, P, ldp, AP, ldap, nullptr, 0, nullptr, denom, w, st);
+  if (s != COH_OK) return s;
+  hipLaunchKernelGGL(k_pcg_alpha, dim3((unsigned)ceil_div(k, 256)), dim3(256), 0, st, k, rs, denom,
+                     active, alpha, breakdown);
+  COH_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_pcg_xr, dim3(ew_grid(n * k)), dim3(256), 0, st, n, k, X, ldx, R, ldr, P, ldp,
+                     AP, ldap, alpha);
+  COH_CHECK_LAUNCH();
+  return COH_OK;
+}
+
+extern "C" int coh_pcg_direction_update(int64_t n, int64_t k, float* P, int64_t ldp,
+                                        const float* R, int64_t ldr, const float* Z, int64_t ldz,
+                                        double* rs, const double* threshold, double* rnorm,
+                                        int32_t* active, void* ws, uint64_t ws_bytes,
+                                        void* stream) {
+  COH_REQUIRE(P && R && Z && rs && threshold && rnorm && active, COH_ERR_VALUE, "NULL argument");
+  COH_REQUIRE(n >= 0 && k >= 1 && ldp >= k && ldr >= k && ldz >= k, COH_ERR_VALUE,
+              "invalid shape");
+  COH_WS_CHECK(ws, ws_bytes, k);
+  hipStream_t st = (hipStream_t)stream;
+  uint8_t* w = static_cast<uint8_t*>(ws);
+  double* scratch = reinterpret_cast<double*>(w + round_up((int64_t)kMaxRedBlocks * k * 8, 256));
+  double* rs_new = scratch;
+  double* rr = scratch + k;
+  double* beta = scratch + 2 * k;
+  int s = coldot(n, k, R, ldr, Z, ldz, nullptr, 0, nullptr, rs_new, w, st);
+  if (s != COH_OK) return s;
+  s = coldot(n, k, R, ldr, R, ldr, nullptr, 0, nullptr, rr, w, st);
+  if (s != COH_OK) return s;
+  hipLaunchKernelGGL(k_pcg_dir_scalars, dim3((unsigned)ceil_div(k, 256)), dim3(256), 0, st, k,
+                     rs_new, rr, rs, threshold, rnorm, active, beta);
+  COH_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_pcg_p, dim3(ew_grid(n * k)), dim3(256), 0, st, n, k, P, ldp, Z, ldz, active,
+                     beta);
+  COH_CHECK_LAUNCH();
+  return COH_OK;
+}
+
+extern "C" int coh_lanczos_step(int64_t n, int64_t k, const float* v, const float* Kv,
+                                const float* prev, const double* beta_prev, float* v_next,
+                                double* alpha, double* beta, void* ws, uint64_t ws_bytes,
+                                void* stream) {
+  COH_REQUIRE(v && Kv && prev && beta_prev && v_next && alpha && beta, COH_ERR_VALUE,
+              "NULL argument");
+  COH_REQUIRE(n >= 0 && k >= 1, COH_ERR_VALUE, "invalid shape");
+  COH_REQUIRE(v_next != v && v_next != Kv, COH_ERR_VALUE, "v_next may alias only prev");
+  COH_WS_CHECK(ws, ws_bytes, k);
+  hipStream_t st = (hipStream_t)stream;
+  uint8_t* w = static_cast<uint8_t*>(ws);
+  double* scratch = reinterpret_cast<double*>(w + round_up((int64_t)kMaxRedBlocks * k * 8, 256));
+  double* ww = scratch;
+  // alpha = colsum(v * (Kv - beta_prev * prev))
+  int s = coldot(n, k, v, k, Kv, k, prev, k, beta_prev, alpha, w, st);
+  if (s != COH_OK) return s;
+  hipLaunchKernelGGL(k_lanczos_w, dim3(ew_grid(n * k)), dim3(256), 0, st, n, k, v, Kv, prev,
+                     beta_prev, alpha, v_next);
+  COH_CHECK_LAUNCH();
+  s = coldot(n, k, v_next, k, v_next, k, nullptr, 0, nullptr, ww, w, st);
+  if (s != COH_OK) return s;
+  hipLaunchKernelGGL(k_sqrt_vec, dim3((unsigned)ceil_div(k, 256)), dim3(256), 0, st, k, ww, beta);
+  COH_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_safe_div, dim3(ew_grid(n * k)), dim3(256), 0, st, n, k, v_next, beta);
+  COH_CHECK_LAUNCH();
+  return COH_OK;
+}

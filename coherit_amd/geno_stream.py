@@ -1,0 +1,565 @@
+"""GenoBlockStreamer: the genotype panel, resident in HBM.
+
+Same constructor and operator methods as the reference streamer
+(geno_stream.py:1147-2864).  The reference builds a 2-bit packed tmpfile
+cache and re-streams it over PCIe on every K.V pass; here the packed panel
+is transcoded on the GPU once and stays resident in two layouts, and every
+operator is one pass of hand-written HIP kernels over HBM.
+"""
+from __future__ import annotations
+
+import dataclasses
+import logging
+import time
+
+import numpy as np
+import torch
+
+from . import _native as N
+from .block_backend import DensePackedBlockDescriptor
+from .panel import ResidentPanel, _dev
+from .sources import BedGenoSource
+from .synth import SyntheticSource
+
+logger = logging.getLogger(__name__)
+
+_INGEST_BUDGET_BYTES = 256 * 1024 * 1024
+
+
+# --------------------------------------------------------------------------
+# geometry (geno_stream.py:364-505)
+# --------------------------------------------------------------------------
+
+@dataclasses.dataclass(frozen=True)
+class _ComponentPartitionPlan:
+    component_sizes: tuple
+    cache_to_source_variant_indices: np.ndarray
+    has_arbitrary_groups: bool
+
+
+def _normalize_component_block_sizes(m, component_block_sizes, *, allow_empty=False):
+    """geno_stream.py:371-392."""
+    if component_block_sizes is None:
+        return (int(m),)
+    sizes = np.asarray(component_block_sizes, dtype=np.int64).reshape(-1)
+    if sizes.size == 0:
+        raise ValueError("component_block_sizes must contain at least one block.")
+    if allow_empty:
+        if np.any(sizes < 0):
+            raise ValueError("component_block_sizes must be non-negative.")
+    elif np.any(sizes <= 0):
+        raise ValueError("component_block_sizes must be strictly positive.")
+    if int(sizes.sum()) != int(m):
+        raise ValueError(f"component_block_sizes must sum to m={int(m)}; got {int(sizes.sum())}.")
+    return tuple(int(x) for x in sizes.tolist())
+
+
+def _normalize_component_variant_indices(m, component_variant_indices):
+    """geno_stream.py:395-437: each group sorted and de-duplicated."""
+    groups = list(component_variant_indices or [])
+    if not groups:
+        raise ValueError("component_variant_indices must contain at least one component.")
+    normalized = []
+    for idx, group in enumerate(groups):
+        arr = np.asarray(group, dtype=np.int64).reshape(-1)
+        if arr.size == 0:
+            normalized.append(arr)
+            continue
+        if np.any((arr < 0) | (arr >= int(m))):
+            raise IndexError(
+                f"component_variant_indices[{idx}] contains out-of-range SNP indices for m={int(m)}."
+            )
+        uniq = np.unique(arr)
+        if uniq.size != arr.size:
+            raise ValueError(f"component_variant_indices[{idx}] contains duplicate SNP indices.")
+        normalized.append(uniq)
+    order = np.concatenate(normalized) if normalized else np.empty((0,), np.int64)
+    if order.size == 0:
+        raise ValueError("component_variant_indices must cover at least one SNP.")
+    if order.size > int(m):
+        raise ValueError(
+            f"component_variant_indices assign {order.size} SNPs, which exceeds source m={int(m)}."
+        )
+    if np.unique(order).size != order.size:
+        raise ValueError("component_variant_indices must be disjoint across components.")
+    if order.size == int(m) and not np.array_equal(np.sort(order), np.arange(int(m))):
+        raise ValueError("component_variant_indices must cover every SNP exactly once.")
+    return tuple(int(a.size) for a in normalized), order
+
+
+def _build_component_partition_plan(m, *, component_block_sizes=None,
+                                    component_variant_indices=None):
+    """geno_stream.py:440-466."""
+    if component_block_sizes is not None and component_variant_indices is not None:
+        raise ValueError("Provide either component_block_sizes or component_variant_indices, not both.")
+    if component_variant_indices is not None:
+        sizes, order = _normalize_component_variant_indices(m, component_variant_indices)
+    else:
+        sizes = _normalize_component_block_sizes(m, component_block_sizes)
+        order = np.arange(int(m), dtype=np.int64)
+    return _ComponentPartitionPlan(
+        sizes, order, not np.array_equal(order, np.arange(int(m), dtype=np.int64))
+    )
+
+
+def _build_call_geometry(m, call_width, component_block_sizes):
+    """geno_stream.py:469-505: calls of <= call_width SNPs within components."""
+    sizes = _normalize_component_block_sizes(m, component_block_sizes, allow_empty=True)
+    starts, widths, comp_ids, offsets = [], [], [], [0]
+    snp = 0
+    for cid, size in enumerate(sizes):
+        rem = int(size)
+        while rem > 0:
+            w = min(int(call_width), rem)
+            starts.append(snp)
+            widths.append(w)
+            comp_ids.append(cid)
+            snp += w
+            rem -= w
+        offsets.append(len(starts))
+    return (np.asarray(starts, np.int64), np.asarray(widths, np.int32),
+            np.asarray(comp_ids, np.int32), np.asarray(offsets, np.int32))
+
+
+def _source_spans(ids: np.ndarray):
+    """Contiguous runs of source SNP ids (geno_stream.py:508-549)."""
+    if ids.size == 0:
+        return []
+    breaks = np.flatnonzero(np.diff(ids) != 1) + 1
+    starts = np.concatenate([[0], breaks])
+    ends = np.concatenate([breaks, [ids.size]])
+    return [(int(ids[a]), int(b - a)) for a, b in zip(starts, ends)]
+
+
+def _standardized_sumsq_from_raw_stats(cnt, s1, s2, mean, inv_sd):
+    """geno_stream.py:1481-1496 (FP64)."""
+    cnt64 = np.asarray(cnt, dtype=np.float64)
+    s1_64 = np.asarray(s1, dtype=np.float64)
+    s2_64 = np.asarray(s2, dtype=np.float64)
+    mean64 = np.asarray(mean, dtype=np.float64)
+    inv64 = np.asarray(inv_sd, dtype=np.float64)
+    centered = s2_64 - 2.0 * mean64 * s1_64 + cnt64 * mean64 * mean64
+    return np.maximum(centered, 0.0) * inv64 * inv64
+
+
+# --------------------------------------------------------------------------
+# streamer
+# --------------------------------------------------------------------------
+
+class GenoBlockStreamer:
+    """Genotype panel resident in HBM with the reference streamer's API.
+
+    Extra keyword ``precision`` selects the contraction kernels: ``"fp32"``
+    (plain, the drop-in), ``"i8x3"`` (exact 24-bit int8 limbs on MFMA) or
+    ``"i8x2"`` (16-bit int8 limbs on MFMA).
+    """
+
+    def __init__(self, source=None, *, bed_prefix=None, call_width=131072,
+                 component_block_sizes=None, component_variant_indices=None,
+                 standardization_override=None, device=None, keep_host_stats=True,
+                 build_threads=None, sample_mask=None, ring_depth=None,
+                 source_build_chunk_width=None, precision="fp32"):
+        if source is not None and bed_prefix is not None:
+            raise ValueError("Provide either source or bed_prefix, not both.")
+        if source is None:
+            if bed_prefix is None:
+                raise ValueError("Either source or bed_prefix must be provided.")
+            source = BedGenoSource(bed_prefix, threads=build_threads, sample_mask=sample_mask)
+            sample_mask = None
+        if precision not in N.PRECISIONS:
+            raise ValueError(f"precision must be one of {sorted(N.PRECISIONS)}, got {precision!r}")
+        self.precision = precision
+        self._prec = N.PRECISIONS[precision]
+        self._source = source
+        self.sample_ids = getattr(source, "sample_ids", None)
+        self._sample_mask = None if sample_mask is None else np.asarray(sample_mask, dtype=bool)
+        if self._sample_mask is not None and self._sample_mask.shape != (int(source.n),):
+            raise ValueError(
+                "sample_mask is incompatible with the supplied source: "
+                f"mask length={self._sample_mask.size}, source.n={int(source.n)}."
+            )
+        if self.sample_ids is not None and self._sample_mask is not None:
+            self.sample_ids = tuple(i for i, k in zip(self.sample_ids, self._sample_mask) if k)
+        self.source_m = int(source.m)
+        self.n = int(self._sample_mask.sum()) if self._sample_mask is not None else int(source.n)
+        self.m = int(source.m)
+        if self.n <= 0:
+            raise ValueError("Genotype source must contain at least one selected sample.")
+        if self.m <= 0:
+            raise ValueError("Genotype source must contain at least one variant.")
+        self._bed_int_missing = int(source.missing_val)
+        self._missing_val = np.uint8(3)
+        self.call_width = max(1, int(call_width))
+        self.keep_host_stats = bool(keep_host_stats)
+        self.dev = _dev(device)
+        self._has_component_partition = (
+            component_block_sizes is not None or component_variant_indices is not None
+        )
+        plan = _build_component_partition_plan(
+            self.source_m, component_block_sizes=component_block_sizes,
+            component_variant_indices=component_variant_indices,
+        )
+        self._component_block_sizes = plan.component_sizes
+        if self._has_component_partition:
+            self.m = int(sum(plan.component_sizes))
+        self._cache_to_source_variant_indices = (
+            plan.cache_to_source_variant_indices if self._has_component_partition else None
+        )
+        self._has_arbitrary_component_partition = bool(
+            self._has_component_partition and plan.has_arbitrary_groups
+        )
+        override = None
+        if standardization_override is not None:
+            mo = np.asarray(standardization_override[0], dtype=np.float32).reshape(-1)
+            io = np.asarray(standardization_override[1], dtype=np.float32).reshape(-1)
+            if mo.size != self.m or io.size != self.m:
+                raise ValueError(
+                    f"standardization_override length mismatch: expected m={self.m}, "
+                    f"got means={mo.size}, inv={io.size}."
+                )
+            override = (mo, io)
+        (self._call_snp_starts, self._call_true_widths, self._call_component_ids,
+         self._component_call_offsets) = _build_call_geometry(
+            self.m, self.call_width, self._component_block_sizes)
+        self._n_calls = int(self._call_snp_starts.size)
+        self._n_components = len(self._component_block_sizes)
+        self._packed_call_widths = ((self._call_true_widths + 3) // 4).astype(np.int32)
+        self._max_packed_width = int(self._packed_call_widths.max())
+        self._max_unpack_width = max(1, 4 * self._max_packed_width)
+        self._component_snp_offsets = np.concatenate(
+            [[0], np.cumsum(self._component_block_sizes)]).astype(np.int64)
+
+        with torch.cuda.device(self.dev):
+            self._panel = ResidentPanel(self.n, self._call_snp_starts, self._call_true_widths,
+                                        self._call_component_ids, self._n_components, self.dev)
+            t0 = time.perf_counter()
+            if isinstance(source, BedGenoSource):
+                self._ingest_bed(source)
+            elif isinstance(source, SyntheticSource):
+                if self._sample_mask is not None:
+                    raise ValueError("sample_mask is not supported for generated panels.")
+                order = (self._cache_to_source_variant_indices
+                         if self._cache_to_source_variant_indices is not None
+                         else np.arange(self.m, dtype=np.int64))
+                source.generate_into(self._panel, order)
+            elif hasattr(source, "read_block_variant_major"):
+                self._ingest_varmaj(source)
+            else:
+                raise NotImplementedError(
+                    "GenoBlockStreamer requires either BedGenoSource or a source implementing "
+                    "read_block_variant_major()."
+                )
+            self._panel.finalize(*(override if override is not None else (None, None)))
+            self.build_seconds = time.perf_counter() - t0
+            self._init_stats(override)
+        close = getattr(source, "close", None)
+        if callable(close):
+            close()
+        self._source = None
+        self._exact_diag_dev = None
+
+    # ------------------------------------------------------------- ingest
+    def _cache_source_ids(self, s0, s1):
+        if self._cache_to_source_variant_indices is None:
+            return np.arange(s0, s1, dtype=np.int64)
+        return self._cache_to_source_variant_indices[s0:s1]
+
+    def _chunk(self, row_bytes):
+        return max(1, min(self.m, _INGEST_BUDGET_BYTES // max(1, row_bytes)))
+
+    def _ingest_varmaj(self, source):
+        chunk = self._chunk(int(source.n))
+        for s0 in range(0, self.m, chunk):
+            s1 = min(self.m, s0 + chunk)
+            ids = self._cache_source_ids(s0, s1)
+            parts = [np.asarray(source.read_block_variant_major(a, w), dtype=np.int8)
+                     for a, w in _source_spans(ids)]
+            block = np.concatenate(parts, axis=0) if len(parts) > 1 else parts[0]
+            if self._sample_mask is not None:
+                block = block[:, self._sample_mask]
+            dev = torch.from_numpy(np.ascontiguousarray(block)).to(self.dev)
+            self._panel.ingest_i8_rows(s0, dev, self._bed_int_missing)
+
+    def _ingest_bed(self, source):
+        bps = source.bytes_per_snp
+        kept = torch.from_numpy(source.kept_sample_index()).to(self.dev)
+        chunk = self._chunk(bps)
+        for s0 in range(0, self.m, chunk):
+            s1 = min(self.m, s0 + chunk)
+            rows = source.read_raw_rows(self._cache_source_ids(s0, s1))
+            dev = torch.from_numpy(rows).to(self.dev)
+            self._panel.ingest_bed_rows(s0, dev, bps, kept)
+        torch.cuda.current_stream(self.dev).synchronize()
+
+    def _init_stats(self, override):
+        """eff / atoms exactly as geno_stream.py:1394-1448."""
+        mean, inv, cnt, s1, s2 = self._panel.stats()
+        if override is not None:
+            trace_mass = _standardized_sumsq_from_raw_stats(cnt, s1, s2, mean, inv)
+            counts_flat = None
+        else:
+            trace_mass = cnt.astype(np.float64)
+            counts_flat = cnt.astype(np.int32)
+        self._means_host = mean if self.keep_host_stats else None
+        self._inv_sds_host = inv if self.keep_host_stats else None
+        self._count_host = counts_flat if self.keep_host_stats else None
+        self._raw_stats_host = (cnt, s1, s2)
+        valid = inv > 0.0
+        eff = float(np.count_nonzero(valid))
+        self._eff_m_host = eff
+        self._mean_diag_atom_host = (
+            float(np.sum(trace_mass[valid], dtype=np.float64) / (float(self.n) * eff))
+            if eff > 0.0 else 0.0
+        )
+        G = self._n_components
+        ceff = np.zeros(G, np.float32)
+        cdiag = np.zeros(G, np.float32)
+        off = self._component_snp_offsets
+        for g in range(G):
+            a, b = int(off[g]), int(off[g + 1])
+            ceff[g] = float(np.count_nonzero(valid[a:b]))
+            if ceff[g] > 0.0:
+                cdiag[g] = float(np.sum(trace_mass[a:b][valid[a:b]], dtype=np.float64)
+                                 / (float(self.n) * float(ceff[g])))
+        self._component_eff_m_host = ceff
+        self._component_mean_diag_atoms_host = cdiag
+        self._eff_m_const = torch.tensor(eff, dtype=torch.float32, device=self.dev)
+        self._component_eff_m_const = torch.as_tensor(ceff, device=self.dev)
+        self._true_widths_dev = torch.as_tensor(self._call_true_widths, device=self.dev)
+        self._snp_starts_dev = torch.as_tensor(self._call_snp_starts.astype(np.int32), device=self.dev)
+        self._block_backend_plan = tuple(
+            DensePackedBlockDescriptor(
+                call_idx=c, snp_start=int(self._call_snp_starts[c]),
+                true_width=int(self._call_true_widths[c]),
+                packed_width=int(self._packed_call_widths[c]),
+                storage="hbm_resident_packed",
+            )
+            for c in range(self._n_calls)
+        )
+
+    # ---------------------------------------------------- compat accessors
+    @property
+    def _means_by_call(self):
+        return self._by_call(self._panel_stats_host()[0])
+
+    @property
+    def _inv_by_call(self):
+        return self._by_call(self._panel_stats_host()[1])
+
+    def _panel_stats_host(self):
+        if self._means_host is not None:
+            return self._means_host, self._inv_sds_host
+        mean, inv = self._panel.stats()[:2]
+        return mean, inv
+
+    def _by_call(self, flat):
+        out = np.zeros((self._n_calls, self._max_unpack_width), dtype=np.float32)
+        for c in range(self._n_calls):
+            s0, tw = int(self._call_snp_starts[c]), int(self._call_true_widths[c])
+            out[c, :tw] = flat[s0 : s0 + tw]
+        return torch.as_tensor(out, device=self.dev)
+
+    def _pop_cached(self, call_idx) -> torch.Tensor:
+        """Packed call block (n, ceil(w/4)) in the reference cache layout."""
+        return self._panel.read_packed_call(int(call_idx))
+
+    def _prepare_kv_pass(self) -> None:
+        """No ring to prime: the panel is resident (geno_stream.py:2631-2633)."""
+        self._panel.handle  # raises when closed
+
+    def block_backend_plan(self):
+        return self._block_backend_plan
+
+    @property
+    def has_sparse_backend(self) -> bool:
+        return False
+
+    @property
+    def has_component_partition(self) -> bool:
+        return self._has_component_partition
+
+    @property
+    def has_arbitrary_component_partition(self) -> bool:
+        return self._has_arbitrary_component_partition
+
+    @property
+    def n_components(self) -> int:
+        return self._n_components
+
+    @property
+    def panel(self) -> ResidentPanel:
+        return self._panel
+
+    def component_source_variant_indices(self, component_idx: int) -> np.ndarray:
+        if not self._has_component_partition or self._cache_to_source_variant_indices is None:
+            raise ValueError("Component source SNP indices require component-partitioned streamer geometry.")
+        self._check_component(component_idx)
+        a = int(self._component_snp_offsets[component_idx])
+        b = int(self._component_snp_offsets[component_idx + 1])
+        return np.asarray(self._cache_to_source_variant_indices[a:b], dtype=np.int64)
+
+    def _check_component(self, component_idx):
+        if component_idx < 0 or component_idx >= self._n_components:
+            raise IndexError(
+                f"component_idx={component_idx} out of range for {self._n_components} components."
+            )
+
+    # ----------------------------------------------------------- operators
+    def _scales(self, normalize, effs):
+        return np.asarray([1.0 / e if (normalize and e > 0) else 1.0 for e in effs], np.float32)
+
+    @staticmethod
+    def _mat(V):
+        squeeze = V.ndim == 1
+        return (V[:, None] if squeeze else V), squeeze
+
+    def kv(self, V, normalize: bool = True):
+        """sum_c Z_c Z_c^T V / eff (geno_stream.py:2669-2679, kv_impl.py:268-326)."""
+        self._prepare_kv_pass()
+        V, sq = self._mat(torch.as_tensor(V, device=self.dev))
+        eff = self._eff_m_host
+        scale = np.full(self._n_components, 1.0 / eff if (normalize and eff > 0) else 1.0, np.float32)
+        out = self._panel.kv(V, scale, 0, self._n_components, precision=self._prec)
+        return out[:, 0] if sq else out
+
+    def component_kv(self, V, component_idx: int, normalize: bool = True):
+        """geno_stream.py:2681-2701, kv_impl.py:1042-1083."""
+        self._check_component(component_idx)
+        self._prepare_kv_pass()
+        V, sq = self._mat(torch.as_tensor(V, device=self.dev))
+        scale = self._scales(normalize, self._component_eff_m_host)
+        out = self._panel.kv(V, scale, component_idx, component_idx + 1, precision=self._prec)
+        return out[:, 0] if sq else out
+
+    def stacked_component_kv(self, V, normalize: bool = True):
+        """(G, n, k) stack of normalised K_g V (kv_impl.py:1136-1187)."""
+        self._prepare_kv_pass()
+        V, sq = self._mat(torch.as_tensor(V, device=self.dev))
+        scale = self._scales(normalize, self._component_eff_m_host)
+        out = self._panel.kv(V, scale, 0, self._n_components, stacked=True, precision=self._prec)
+        return out[:, :, 0] if sq else out
+
+    def weighted_component_hv(self, theta_g, theta_e, V):
+        """theta_e V + sum_g theta_g/eff_g Z_g Z_g^T V (kv_impl.py:1086-1133)."""
+        th = _host_vector(theta_g)
+        if th.size != self._n_components:
+            raise ValueError(
+                f"theta_g length mismatch: expected {self._n_components}, got {th.size}."
+            )
+        self._prepare_kv_pass()
+        V, sq = self._mat(torch.as_tensor(V, device=self.dev))
+        V = V.to(torch.float32)
+        effs = self._component_eff_m_host
+        scale = np.asarray([t / e if e > 0 else 0.0 for t, e in zip(th, effs)], np.float32)
+        if theta_e is None:
+            out = self._panel.kv(V, scale, 0, self._n_components, precision=self._prec)
+        else:
+            te = float(_host_vector(theta_e).reshape(-1)[0])
+            out = (V * te).contiguous()
+            self._panel.kv(V, scale, 0, self._n_components, out=out, beta=1.0, precision=self._prec)
+        return out[:, 0] if sq else out
+
+    def xtv(self, V, normalize: bool = False):
+        """(m, k) = Z^T V [/eff] in cache order (kv_impl.py:382-439)."""
+        self._prepare_kv_pass()
+        V, sq = self._mat(torch.as_tensor(V, device=self.dev))
+        eff = self._eff_m_host
+        scale = 1.0 / eff if (normalize and eff > 0) else 1.0
+        out = self._panel.xtv(V, 0, self._n_components, scale=scale, precision=self._prec)
+        return out[:, 0] if sq else out
+
+    def _slot_xtv(self, U):
+        return self._panel.xtv(U, 0, self._n_components, slot_space=True, precision=self._prec)
+
+    def build_projected_core_atom(self, U, *, subtract_identity: bool = True):
+        """(X^T U)^T (X^T U) / eff [- I] (kv_impl.py:1190-1232)."""
+        self._prepare_kv_pass()
+        U = torch.as_tensor(U, device=self.dev)
+        if U.ndim != 2:
+            raise ValueError("build_projected_core_atom_streamed expects U with shape (n, k).")
+        T = self._slot_xtv(U)
+        core = T.T @ T
+        return _finalize_core(core, self._eff_m_host, subtract_identity)
+
+    def build_projected_core_atoms(self, U, *, subtract_identity: bool = True):
+        """Per-component cores T_g = U^T K_g U [- I] (kv_impl.py:1287-1339)."""
+        if not self._has_component_partition:
+            raise ValueError("Projected-core atoms require component-partitioned streamer geometry.")
+        self._prepare_kv_pass()
+        U = torch.as_tensor(U, device=self.dev)
+        if U.ndim != 2:
+            raise ValueError("build_projected_core_atoms_partitioned expects U with shape (n, k).")
+        T = self._slot_xtv(U)
+        base = int(self._panel.comp_slot_begin[0])
+        cores = []
+        for g in range(self._n_components):
+            a = int(self._panel.comp_slot_begin[g]) - base
+            b = int(self._panel.comp_slot_end[g]) - base
+            Tg = T[a:b]
+            cores.append(_finalize_core(Tg.T @ Tg, float(self._component_eff_m_host[g]),
+                                        subtract_identity))
+        return torch.stack(cores)
+
+    def diag(self):
+        """Mean diagonal atom tr(K)/n (geno_stream.py:2773-2778)."""
+        return torch.tensor(self._mean_diag_atom_host, dtype=torch.float32, device=self.dev)
+
+    def component_diag_list(self):
+        return [torch.tensor(float(a), dtype=torch.float32, device=self.dev)
+                for a in self._component_mean_diag_atoms_host]
+
+    def exact_diag(self, *, target_block_bytes: int = 256 * 1024 * 1024):
+        """Per-sample diagonal of K (geno_stream.py:2780-2835)."""
+        if self._exact_diag_dev is None:
+            self._exact_diag_dev = self._panel.exact_diag(self._eff_m_host)
+        return self._exact_diag_dev
+
+    def extract_standardized_columns(self, snp_indices) -> np.ndarray:
+        """(n, len) float32 (g_imp - mean) * inv for cache SNPs
+        (geno_stream.py:2472-2503, 2741-2756)."""
+        idx = np.asarray(snp_indices, dtype=np.int64).reshape(-1)
+        if self._means_host is None:
+            raise RuntimeError("extract_standardized_columns requires keep_host_stats=True.")
+        if idx.size and (idx.min() < 0 or idx.max() >= self.m):
+            raise IndexError("SNP index out of range.")
+        out = np.empty((self.n, idx.size), dtype=np.float32)
+        calls = np.searchsorted(self._call_snp_starts, idx, side="right") - 1
+        for c in np.unique(calls):
+            sel = np.flatnonzero(calls == c)
+            local = idx[sel] - int(self._call_snp_starts[c])
+            block = self._pop_cached(int(c)).cpu().numpy()
+            g = ((block[:, local >> 2] >> ((local & 3) << 1).astype(np.uint8)) & 3).astype(np.float32)
+            mean = self._means_host[idx[sel]][None, :]
+            inv = self._inv_sds_host[idx[sel]][None, :]
+            g_imp = np.where(g == 3, mean, g)
+            out[:, sel] = (g_imp - mean) * inv
+        return out
+
+    def close(self) -> None:
+        panel = getattr(self, "_panel", None)
+        if panel is not None:
+            panel.close()
+
+
+class BedBlockStreamer(GenoBlockStreamer):
+    """Backward-compatible alias: ``BedBlockStreamer(bed_prefix, ...)``."""
+
+    def __init__(self, bed_prefix: str, **kwargs):
+        super().__init__(bed_prefix=bed_prefix, **kwargs)
+
+
+def _host_vector(x) -> np.ndarray:
+    if isinstance(x, torch.Tensor):
+        return x.detach().to("cpu", torch.float64).numpy().reshape(-1)
+    return np.asarray(x, dtype=np.float64).reshape(-1)
+
+
+def _finalize_core(core, eff, subtract_identity):
+    """kv_impl.py:198-218."""
+    if eff > 0:
+        core = core / eff
+        if subtract_identity:
+            core = core - torch.eye(core.shape[0], dtype=core.dtype, device=core.device)
+    else:
+        core = torch.zeros_like(core)
+    return core

@@ -1,0 +1,224 @@
+"""Projected-core preconditioner ``M = d I + U C U^T`` (precond.py:1-290).
+
+The tall-skinny products with the shared basis U (n, r) are plain FP32
+library GEMMs; the small r x r side (Cholesky, triangular solves, eigh) is
+FP64 on the device.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+from typing import Callable, Optional
+
+import numpy as np
+import torch
+
+
+@dataclass
+class ProjectedCorePrecondConf:
+    """precond.py:15-27."""
+
+    U: torch.Tensor                        # (n, r) orthonormal basis
+    core_atoms: torch.Tensor               # (G, r, r)
+    total_rank: int
+    n_grm: int
+    diag_mode: Optional[str] = None        # "scalar_identity"
+    diag_atoms: Optional[torch.Tensor] = None
+    residual_diag_atoms: Optional[torch.Tensor] = None
+    identity: Optional[torch.Tensor] = None
+
+
+@dataclass
+class ProjectedCoreRuntime:
+    """precond.py:30-39 (chol/eig factors kept in FP64)."""
+
+    U: torch.Tensor
+    total_rank: int
+    d: float
+    d_inv: float
+    chol: torch.Tensor
+    eigvals: Optional[torch.Tensor] = None
+    eigvecs: Optional[torch.Tensor] = None
+
+
+def _apply_scalar_diag(v, d_inv, chol, U):
+    """precond.py:42-58: M^{-1} v = d^{-1} v + U((dI+C)^{-1} - d^{-1}) U^T v."""
+    vec = v.ndim == 1
+    V = v[:, None] if vec else v
+    rhs = U.T @ V                                   # (r, k) FP32 GEMM
+    rhs64 = rhs.double()
+    z = torch.cholesky_solve(rhs64, chol, upper=False)
+    corr = (z - d_inv * rhs64).to(torch.float32)
+    out = d_inv * V + U @ corr
+    return out[:, 0] if vec else out
+
+
+def _apply_scalar_diag_invsqrt(v, d_inv_sqrt, eigvals, eigvecs, U):
+    """precond.py:61-84: M^{-1/2} v via the eigenpairs of d I + C."""
+    vec = v.ndim == 1
+    V = v[:, None] if vec else v
+    rhs = (U.T @ V).double()
+    coeff = eigvecs.T @ rhs
+    scaled = coeff / torch.sqrt(torch.clamp(eigvals, min=1e-8))[:, None]
+    z = eigvecs @ scaled
+    out = d_inv_sqrt * V + U @ (z - d_inv_sqrt * rhs).to(torch.float32)
+    return out[:, 0] if vec else out
+
+
+def _scalar_diag_value(diag_H) -> float:
+    arr = np.asarray(diag_H.detach().cpu() if isinstance(diag_H, torch.Tensor) else diag_H,
+                     dtype=np.float64)
+    if arr.ndim == 0:
+        return float(arr)
+    if arr.size != 1:
+        raise ValueError("Projected-core MVP currently requires a scalar diagonal base.")
+    return float(arr.reshape(()))
+
+
+def _vec(x) -> np.ndarray:
+    if isinstance(x, torch.Tensor):
+        return x.detach().cpu().double().numpy().reshape(-1)
+    return np.asarray(x, dtype=np.float64).reshape(-1)
+
+
+def scalar_diag_from_precond_conf(precond_conf, theta_g, theta_e) -> Optional[float]:
+    """precond.py:97-118: d = theta_e . residual_atoms + theta_g . diag_atoms."""
+    if precond_conf is None or precond_conf.diag_mode != "scalar_identity":
+        return None
+    tg = _vec(theta_g)
+    te = _vec(theta_e)
+    diag_atoms = (np.ones(precond_conf.n_grm) if precond_conf.diag_atoms is None
+                  else _vec(precond_conf.diag_atoms))
+    if precond_conf.residual_diag_atoms is None:
+        res = np.ones(te.size)
+    else:
+        res = _vec(precond_conf.residual_diag_atoms)
+        if res.size != te.size:
+            raise ValueError(
+                "Projected-core residual_diag_atoms length mismatch: "
+                f"expected {te.size}, got {res.size}."
+            )
+    return float(np.dot(te, res) + np.dot(tg, diag_atoms))
+
+
+def build_projected_core_runtime(precond_conf, theta_g, diag_H, eps: float = 0.0, *,
+                                 need_invsqrt: bool = False) -> Optional[ProjectedCoreRuntime]:
+    """precond.py:121-171."""
+    if precond_conf is None:
+        return None
+    if precond_conf.U.numel() == 0 or precond_conf.total_rank <= 0:
+        return None
+    if precond_conf.diag_mode != "scalar_identity":
+        raise NotImplementedError(
+            "Projected-core preconditioner currently supports only diag_mode='scalar_identity'."
+        )
+    U = precond_conf.U
+    tg = torch.as_tensor(_vec(theta_g), device=U.device)
+    core = torch.tensordot(tg, precond_conf.core_atoms.double(), dims=1)
+    core = 0.5 * (core + core.T)
+    d = max(_scalar_diag_value(diag_H) + float(eps), 1e-8)
+    r = precond_conf.total_rank
+    if precond_conf.identity is not None and tuple(precond_conf.identity.shape) != (r, r):
+        raise ValueError(
+            f"Projected-core identity cache shape mismatch: expected {(r, r)}, "
+            f"got {tuple(precond_conf.identity.shape)}."
+        )
+    middle = core + d * torch.eye(r, dtype=torch.float64, device=U.device)
+    chol, info = torch.linalg.cholesky_ex(middle)
+    if int(info) != 0:
+        raise FloatingPointError("Projected-core middle matrix is not positive definite.")
+    eigvals = eigvecs = None
+    if need_invsqrt:
+        ev, eigvecs = torch.linalg.eigh(middle)
+        eigvals = torch.clamp(ev, min=1e-8)
+    return ProjectedCoreRuntime(U=U, total_rank=r, d=d, d_inv=1.0 / d, chol=chol,
+                                eigvals=eigvals, eigvecs=eigvecs)
+
+
+def make_projected_core_precond_from_runtime(runtime) -> Optional[Callable]:
+    """precond.py:174-186."""
+    if runtime is None:
+        return None
+
+    def apply(v):
+        return _apply_scalar_diag(v, runtime.d_inv, runtime.chol, runtime.U)
+
+    return apply
+
+
+def projected_core_apply_invsqrt(runtime, v):
+    """precond.py:189-204."""
+    if runtime.eigvals is None or runtime.eigvecs is None:
+        raise ValueError(
+            "Projected-core runtime missing eigendecomposition required for inverse-square-root apply."
+        )
+    return _apply_scalar_diag_invsqrt(v, float(np.sqrt(runtime.d_inv)), runtime.eigvals,
+                                      runtime.eigvecs, runtime.U)
+
+
+def projected_core_logdet(runtime, n_dim: int) -> float:
+    """precond.py:207-221: (n - r) log d + 2 sum log diag L."""
+    if n_dim < runtime.total_rank:
+        raise ValueError(
+            f"n_dim={n_dim} is smaller than projected-core rank {runtime.total_rank}."
+        )
+    diag = torch.diagonal(runtime.chol)
+    tiny = torch.finfo(diag.dtype).tiny
+    return float((n_dim - runtime.total_rank) * np.log(runtime.d)
+                 + 2.0 * torch.log(torch.clamp(diag, min=tiny)).sum())
+
+
+def make_projected_core_precond(precond_conf, theta_g, diag_H, eps: float = 0.0):
+    runtime = build_projected_core_runtime(precond_conf, theta_g, diag_H, eps=eps,
+                                           need_invsqrt=False)
+    return make_projected_core_precond_from_runtime(runtime)
+
+
+def make_precond(precond_conf, theta_g, diag_H, eps: float = 0.0):
+    return make_projected_core_precond(precond_conf, theta_g, diag_H, eps=eps)
+
+
+def gaussian_sketch(n: int, r: int, seed: int, device) -> torch.Tensor:
+    """Seeded N(0,1) sketch (n, r) generated on the device."""
+    gen = torch.Generator(device=device)
+    gen.manual_seed(int(seed))
+    return torch.randn((n, r), generator=gen, device=device, dtype=torch.float32)
+
+
+def build_lowrank_basis(K_mv: Callable, n: int, max_rank: int, key, oversample: int = 8,
+                        dtype=torch.float32, omega: Optional[torch.Tensor] = None):
+    """Nystrom-style basis, two passes over K (precond.py:251-276).
+
+    ``key`` is an integer seed for the device Gaussian sketch (the reference
+    takes a JAX PRNGKey); ``omega`` may supply the sketch explicitly.
+    Returns ``(U (n, k), evals (k,))`` with evals sorted descending, >= 0.
+    """
+    r = int(max_rank) + int(oversample)
+    if omega is None:
+        dev = torch.device("cuda", torch.cuda.current_device())
+        omega = gaussian_sketch(n, r, int(key), dev)
+    Y = K_mv(omega)
+    Q, _ = torch.linalg.qr(Y, mode="reduced")
+    Z = K_mv(Q)
+    Bm = (Q.T @ Z).double()
+    Bm = 0.5 * (Bm + Bm.T)
+    evals, evecs = torch.linalg.eigh(Bm)
+    idx = torch.argsort(evals, descending=True)
+    evals = torch.clamp(evals[idx], min=0.0)
+    evecs = evecs[:, idx]
+    k = min(int(max_rank), int(Q.shape[1]))
+    U = Q @ evecs[:, :k].to(torch.float32)
+    return U, evals[:k].to(torch.float32)
+
+
+__all__ = [
+    "ProjectedCorePrecondConf",
+    "ProjectedCoreRuntime",
+    "build_projected_core_runtime",
+    "make_projected_core_precond",
+    "make_projected_core_precond_from_runtime",
+    "make_precond",
+    "projected_core_apply_invsqrt",
+    "projected_core_logdet",
+    "scalar_diag_from_precond_conf",
+    "build_lowrank_basis",
+]

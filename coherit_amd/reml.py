@@ -1,0 +1,1031 @@
+"""REML fitting via AI-REML / Fisher scoring (reml.py).
+
+Host orchestration mirrors the reference's strict optimiser: objective =
+-(y'Py + log|H| + log|X'H^-1X|)/(2n) with fixed-probe SLQ log-determinants
+(affine single-GRM cache or multi-GRM value+gradient), AI matrix
+0.5 GPy' P GPy / n, a bound-constrained NNLS Fisher step (SciPy, FP64 on the
+host exactly as reml.py:979-1131) and a halving line search.  Every genotype
+pass goes through the HIP operator kernels; PCG and Lanczos updates are HIP
+kernels; small dense algebra (p x p, (G+E)^2, tridiagonal eigensolves) is
+FP64.  Parameters are held on the host in FP64.
+
+Probe streams: the reference draws Rademacher probes from JAX PRNG keys; here
+``seed`` drives PCG64 streams (``slq.rademacher_probes``).
+"""
+from __future__ import annotations
+
+import logging
+import math
+import time
+from dataclasses import dataclass
+from datetime import datetime
+from typing import Callable, Optional, Sequence
+
+import numpy as np
+import scipy.linalg as sla
+import torch
+from scipy.optimize import nnls
+
+from . import _native as N
+from .pcg import pcg_solve, _red_ws
+from .precond import (
+    ProjectedCorePrecondConf,
+    build_projected_core_runtime,
+    make_precond,
+    make_projected_core_precond_from_runtime,
+    projected_core_apply_invsqrt,
+    projected_core_logdet,
+    scalar_diag_from_precond_conf,
+)
+from .slq import EPS32, logdet_value_and_grad as _slq_value_and_grad, rademacher_probes
+
+logger = logging.getLogger(__name__)
+
+FI_SYSTEM_RIDGE = 1e-4
+
+
+@dataclass(frozen=True)
+class AffineSLQCache:
+    """Lanczos coefficients for theta_g K + theta_e I (reml.py:52-59)."""
+
+    alphas_k: np.ndarray     # (m, S) FP64
+    betas_k: np.ndarray      # (m-1, S)
+    z_norm_sq: float
+    nsamples_f: float
+
+
+@dataclass
+class REMLProbeCache:
+    """Response/theta-independent work reused across fits (reml.py:62-74)."""
+
+    signature: tuple | None = None
+    kvrand_stack: object = None
+    affine_slq: AffineSLQCache | None = None
+
+
+@dataclass
+class REMLContext:
+    n: int
+    G: int
+    E: int
+    K_mvs: tuple
+    weighted_hv: Optional[Callable]
+    stacked_kv: Optional[Callable]
+    diag_stack: Optional[torch.Tensor]
+    residual_diag_stack: Optional[torch.Tensor]
+    xmat: Optional[torch.Tensor]
+    y: torch.Tensor
+    rhs_const: torch.Tensor
+    y_col: int
+    rand_stop: int
+    n_XyZ_cols: int
+    R_rand: int
+    precond_conf: Optional[ProjectedCorePrecondConf]
+    diag_atoms: Optional[np.ndarray] = None
+    residual_diag_atoms: Optional[np.ndarray] = None
+    affine_slq_cache: Optional[AffineSLQCache] = None
+    slq_reference_runtime: object = None
+    slq_workspace_bytes: int = 256 * 1024**2
+    counters: dict | None = None
+
+
+@dataclass
+class AverageInfoMatrix:
+    mat: np.ndarray
+    ridge: float = FI_SYSTEM_RIDGE
+    stats: Optional["FisherSolveStats"] = None
+
+
+@dataclass
+class FisherSolveStats:
+    free_dim: int = 0
+    frozen_genetic: int = 0
+    ai_pcg_iters: int = 0
+    ai_elapsed_sec: float = 0.0
+    ws_resolve_count: int = 0
+    ws_fixed_total: int = 0
+    ws_released_total: int = 0
+    ws_trace: str = ""
+
+    def to_dict(self):
+        return {k: getattr(self, k) for k in (
+            "free_dim", "frozen_genetic", "ai_pcg_iters", "ai_elapsed_sec", "ws_resolve_count",
+            "ws_fixed_total", "ws_released_total", "ws_trace")}
+
+
+def _dev():
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def standardize_response(y):
+    """reml.py:151-163: (y - mean) / (std + 1e-6), population std, FP32."""
+    y = torch.as_tensor(y, dtype=torch.float32).reshape(-1)
+    if y.numel() == 0:
+        raise ValueError("Phenotype must contain at least one sample.")
+    if not bool(torch.isfinite(y).all()):
+        raise ValueError("Phenotype contains non-finite values.")
+    y_mean = y.mean()
+    y_std = y.std(unbiased=False)
+    if not bool(torch.isfinite(y_std)) or float(y_std) <= 0.0:
+        raise ValueError("Phenotype must have positive finite variance.")
+    y_scale = y_std + 1e-6
+    return (y - y_mean) / y_scale, y_mean, y_scale
+
+
+def _validate_fixed_effect_design(xmat, relative_tol: float = 1e-7):
+    """reml.py:166-184 (FP64 host)."""
+    gram = (xmat.double().T @ xmat.double()).cpu().numpy()
+    norms_sq = np.diag(gram)
+    if not np.all(np.isfinite(norms_sq)) or np.any(norms_sq <= 0.0):
+        raise ValueError("covar contains a zero or non-finite fixed-effect column.")
+    norms = np.sqrt(norms_sq)
+    corr = gram / (norms[:, None] * norms[None, :])
+    eig = np.linalg.eigvalsh(0.5 * (corr + corr.T))
+    if eig[0] <= float(relative_tol) * max(float(eig[-1]), 1.0):
+        raise ValueError(
+            "covar is rank-deficient or numerically collinear; remove redundant fixed-effect columns."
+        )
+
+
+def _as_host(x) -> np.ndarray:
+    if isinstance(x, torch.Tensor):
+        return x.detach().cpu().numpy()
+    return np.asarray(x)
+
+
+def _mean_diag_atoms(diag_list, *, n_samples=None) -> np.ndarray:
+    """reml.py:187-212: validate diagonals and return tr(K_i)/n."""
+    atoms, ok = [], True
+    for idx, d in enumerate(diag_list):
+        arr = np.asarray(_as_host(d), dtype=np.float32)
+        if arr.ndim not in (0, 1):
+            raise ValueError(f"diag_list[{idx}] must be scalar or one-dimensional.")
+        if arr.ndim == 1 and n_samples is not None and arr.size != int(n_samples):
+            raise ValueError(f"diag_list[{idx}] length mismatch: expected {int(n_samples)}, got {arr.size}.")
+        atoms.append(float(arr) if arr.ndim == 0 else float(np.mean(arr)))
+        ok = ok and bool(np.all(np.isfinite(arr)) and np.all(arr >= 0.0))
+    if not atoms:
+        return np.zeros((0,), np.float64)
+    if not ok:
+        raise ValueError("diag_list entries must contain finite nonnegative values.")
+    return np.asarray(atoms, dtype=np.float64)
+
+
+def _scalar_diag_from_diag_list(diag_list) -> Optional[np.ndarray]:
+    """reml.py:215-233."""
+    atoms = []
+    for d in diag_list:
+        arr = np.asarray(_as_host(d), dtype=np.float32)
+        if arr.size == 0:
+            atoms.append(0.0)
+            continue
+        if arr.ndim == 0:
+            atoms.append(float(arr))
+            continue
+        flat = arr.reshape(-1)
+        if not np.all(flat == flat[0]):
+            return None
+        atoms.append(float(flat[0]))
+    return np.asarray(atoms, dtype=np.float64)
+
+
+def _stable_cho_factor_spd(A: np.ndarray, *, base_jitter=1e-6, max_tries=8):
+    """reml.py:240-277: Cholesky with geometric jitter (FP64 host)."""
+    A = 0.5 * (A + A.T)
+    p = A.shape[0]
+    if p == 0:
+        raise ValueError("stable_cho_factor_spd requires a non-empty square matrix.")
+    scale = float(np.mean(np.abs(np.diag(A))))
+    if not math.isfinite(scale) or scale <= 0.0:
+        scale = 1.0
+    for k in range(-1, max_tries):
+        Atry = A if k < 0 else A + base_jitter * scale * (10.0 ** k) * np.eye(p)
+        try:
+            c = sla.cho_factor(Atry, lower=True, check_finite=False)
+        except np.linalg.LinAlgError:
+            continue
+        d = np.diag(c[0])
+        if np.all(np.isfinite(d)) and np.all(d > 0):
+            return c
+    raise FloatingPointError("Failed to stabilize XtHinvX Cholesky with jitter escalation.")
+
+
+# --------------------------------------------------------------------------
+# SLQ (affine single-GRM) — Lanczos steps are HIP kernels
+# --------------------------------------------------------------------------
+
+def _lanczos_step(v, w_raw, prev, beta_prev, v_next):
+    """reml.py:284-304 via coh_lanczos_step (FP64 alpha/beta)."""
+    n, k = v.shape
+    ws, nb = _red_ws(k, v.device)
+    alpha = torch.empty(k, dtype=torch.float64, device=v.device)
+    beta = torch.empty(k, dtype=torch.float64, device=v.device)
+    w_raw = w_raw.to(torch.float32).contiguous()
+    N.call("coh_lanczos_step", n, k, N.ptr(v), N.ptr(w_raw), N.ptr(prev), N.ptr(beta_prev),
+           N.ptr(v_next), N.ptr(alpha), N.ptr(beta), N.ptr(ws), nb, N.stream_ptr())
+    return v_next, alpha, beta
+
+
+def _tridiag(alphas, betas):
+    m, S = alphas.shape
+    T = np.zeros((S, m, m))
+    idx = np.arange(m)
+    T[:, idx, idx] = alphas.T
+    if m > 1:
+        off = np.arange(m - 1)
+        T[:, off + 1, off] = betas.T
+        T[:, off, off + 1] = betas.T
+    return T
+
+
+def _slq_tridiag_logdet(alphas, betas, z_norm_sq, nsamples_f):
+    """reml.py:307-335 (FP64 host eigensolve)."""
+    m = alphas.shape[0]
+    evals, evecs = np.linalg.eigh(_tridiag(alphas, betas))
+    floor = max(EPS32 * float(m), 1e-6)
+    e1 = evecs[:, 0, :]
+    ests = z_norm_sq * np.sum(e1 * e1 * np.log(np.clip(evals, floor, None)), axis=1)
+    return float(np.sum(ests) / nsamples_f)
+
+
+def _build_affine_slq_cache(K_fn, n_dim, key, *, nsamples, m) -> AffineSLQCache:
+    """reml.py:367-412: Lanczos on K once (no re-orthogonalisation)."""
+    dev = _dev()
+    z = torch.as_tensor(rademacher_probes(key, n_dim, nsamples), device=dev)
+    z_norm = torch.sqrt(torch.tensor(float(n_dim), dtype=torch.float32, device=dev))
+    v = (z / z_norm).T.contiguous()
+    prev = torch.zeros_like(v)
+    spare = torch.empty_like(v)
+    beta_prev = torch.zeros(nsamples, dtype=torch.float64, device=dev)
+    alphas, betas = [], []
+    for i in range(m):
+        w_raw = K_fn(v)
+        v_next, alpha, norm_w = _lanczos_step(v, w_raw, prev, beta_prev, spare)
+        alphas.append(alpha)
+        if i < m - 1:
+            betas.append(norm_w)
+        spare = prev
+        prev, v = v, v_next
+        beta_prev = norm_w
+    A = torch.stack(alphas).cpu().numpy()
+    B = torch.stack(betas).cpu().numpy() if betas else np.empty((0, nsamples))
+    return AffineSLQCache(A, B, float(n_dim), float(nsamples))
+
+
+def _affine_slq_logdet(cache: AffineSLQCache, theta_g, theta_e, *, return_derivatives=False):
+    """reml.py:415-449: T_H = theta_g T_K + theta_e I."""
+    scale = float(theta_g)
+    shift = float(theta_e)
+    if not return_derivatives:
+        return _slq_tridiag_logdet(scale * cache.alphas_k + shift, scale * cache.betas_k,
+                                   cache.z_norm_sq, cache.nsamples_f)
+    return _affine_slq_logdet_derivatives(cache, scale, shift)
+
+
+def _affine_slq_logdet_derivatives(cache, scale, shift):
+    """reml.py:452-486: logdet and its exact (theta_g, theta_e) derivatives."""
+    m = cache.alphas_k.shape[0]
+    evals_k, evecs_k = np.linalg.eigh(_tridiag(cache.alphas_k, cache.betas_k))
+    w = evecs_k[:, 0, :] ** 2
+    raw = scale * evals_k + shift
+    floor = max(EPS32 * float(m), 1e-6)
+    evals_h = np.clip(raw, floor, None)
+    active = (raw > floor).astype(np.float64)
+    factor = cache.z_norm_sq / cache.nsamples_f
+    logdet = factor * np.sum(w * np.log(evals_h))
+    d_scale = factor * np.sum(w * active * evals_k / evals_h)
+    d_shift = factor * np.sum(w * active / evals_h)
+    return float(logdet), np.array([d_scale, d_shift])
+
+
+# --------------------------------------------------------------------------
+# operators
+# --------------------------------------------------------------------------
+
+def _hv_apply(K_mvs, theta_g, theta_e, V, residual_diag_stack=None):
+    """reml.py:547-565: H V = sum_i theta_g[i] K_i V + residual terms."""
+    if residual_diag_stack is None:
+        acc = float(theta_e[0]) * V
+    else:
+        Vm = V[:, None] if V.ndim == 1 else V
+        rd = torch.tensordot(torch.as_tensor(theta_e, dtype=torch.float32, device=V.device),
+                             residual_diag_stack, dims=1)
+        acc = rd[:, None] * Vm
+        if V.ndim == 1:
+            acc = acc[:, 0]
+    for i, mv in enumerate(K_mvs):
+        acc = acc + float(theta_g[i]) * mv(V)
+    return acc
+
+
+def _apply_genetic_stack(ctx, V, component_idx=None):
+    """reml.py:568-583: (G_sel, n, rhs) stacked genetic products."""
+    idx = np.arange(ctx.G) if component_idx is None else np.asarray(component_idx)
+    if idx.size == 0:
+        return torch.zeros((0, V.shape[0], V.shape[1]), dtype=V.dtype, device=V.device)
+    if idx.size == ctx.G and ctx.stacked_kv is not None:
+        return ctx.stacked_kv(V)
+    return torch.stack([ctx.K_mvs[int(i)](V) for i in idx])
+
+
+def _apply_residual_stack(ctx, V):
+    """reml.py:586-593."""
+    Vm = V[:, None] if V.ndim == 1 else V
+    if ctx.residual_diag_stack is None:
+        out = Vm[None]
+    else:
+        out = ctx.residual_diag_stack[:, :, None] * Vm[None]
+    return out[:, :, 0] if V.ndim == 1 else out
+
+
+def _residual_diag_from_components(ctx, theta_e):
+    te = np.asarray(theta_e, dtype=np.float64).reshape(-1)
+    if ctx.residual_diag_stack is None:
+        return float(te[0])
+    return torch.tensordot(torch.as_tensor(te, dtype=torch.float32, device=ctx.y.device),
+                           ctx.residual_diag_stack, dims=1)
+
+
+def _multi_slq_logdet_and_score(ctx, Hv_fn, pvec, key, *, nsamples, m, use_reference,
+                                accept_logdet=None):
+    """reml.py:489-540: whitened fixed-probe SLQ with its streamed pullback."""
+    reference = ctx.slq_reference_runtime if use_reference else None
+
+    def whiten(v):
+        return projected_core_apply_invsqrt(reference, v) if reference is not None else v
+
+    def Bv(v):
+        return whiten(Hv_fn(whiten(v)))
+
+    def pullback(left, right):
+        left, right = whiten(left), whiten(right)
+        result = torch.zeros_like(left)
+        scores = []
+        gen = _apply_genetic_stack(ctx, left)
+        for k in range(ctx.G):
+            result = result + float(pvec[k]) * gen[k]
+            scores.append(float((right.double() * gen[k].double()).sum()))
+        for e in range(ctx.E):
+            prod = left if ctx.residual_diag_stack is None else ctx.residual_diag_stack[e, :, None] * left
+            result = result + float(pvec[ctx.G + e]) * prod
+            scores.append(float((right.double() * prod.double()).sum()))
+        return whiten(result), np.asarray(scores)
+
+    ref_logdet = projected_core_logdet(reference, ctx.n) if reference is not None else 0.0
+    value, gradient = _slq_value_and_grad(
+        Bv, pullback, ctx.n, key, nsamples=nsamples, m=m,
+        workspace_bytes=ctx.slq_workspace_bytes,
+        accept_value=(None if accept_logdet is None
+                      else (lambda v: accept_logdet(v + ref_logdet))),
+    )
+    return value + ref_logdet, gradient
+
+
+# --------------------------------------------------------------------------
+# one objective evaluation
+# --------------------------------------------------------------------------
+
+def _dot64(a, b):
+    return (a.double() * b.double()).sum(0)
+
+
+def _eval_once(ctx, pvec, warm_all, warm_ai=None, *, key_slq, minq_tol, maxiter, precond_eps,
+               slq_samples, slq_m, slq_mode="raw", warm_ready=False, warm_ai_ready=False,
+               compute_traces=True, min_loglik=None):
+    """reml.py:685-971: loglik, score, AI for one parameter vector."""
+    pvec = np.asarray(pvec, dtype=np.float64)
+    theta_g = pvec[: ctx.G]
+    theta_e = pvec[ctx.G:]
+    counters = ctx.counters if ctx.counters is not None else {}
+
+    def Hv(V):
+        counters["hv"] = counters.get("hv", 0) + 1
+        if ctx.weighted_hv is not None:
+            te = theta_e[0] if ctx.E == 1 else theta_e
+            return ctx.weighted_hv(theta_g, te, V)
+        return _hv_apply(ctx.K_mvs, theta_g, theta_e, V, ctx.residual_diag_stack)
+
+    diag_H = scalar_diag_from_precond_conf(ctx.precond_conf, theta_g, theta_e)
+    if diag_H is None:
+        if ctx.diag_atoms is not None:
+            diag_H = _residual_diag_from_components(ctx, theta_e) + float(np.dot(theta_g, ctx.diag_atoms))
+        else:
+            if ctx.diag_stack is None:
+                raise ValueError(
+                    "diag_stack is required when the preconditioner does not provide a scalar diagonal."
+                )
+            diag_H = _residual_diag_from_components(ctx, theta_e) + torch.tensordot(
+                torch.as_tensor(theta_g, dtype=torch.float32, device=ctx.y.device), ctx.diag_stack, dims=1)
+    use_residual_slq = bool(slq_mode == "projected_core_residual" and ctx.affine_slq_cache is None
+                            and ctx.slq_reference_runtime is not None)
+    runtime = build_projected_core_runtime(ctx.precond_conf, theta_g, diag_H, eps=precond_eps)
+    if runtime is not None:
+        M_cur = make_projected_core_precond_from_runtime(runtime)
+    else:
+        M_cur = make_precond(ctx.precond_conf, theta_g, diag_H, eps=precond_eps)
+
+    rhs_all = ctx.rhs_const
+    if warm_ready:
+        X0 = warm_all
+    elif M_cur is not None:
+        X0 = M_cur(rhs_all)
+    else:
+        X0 = torch.zeros_like(rhs_all)
+    sol_all, main_rel, k_pcg = pcg_solve(Hv, rhs_all, M=M_cur, tol=minq_tol, maxiter=maxiter, X0=X0)
+    del X0
+    if not math.isfinite(main_rel) or main_rel > minq_tol * 1.05:
+        raise FloatingPointError(
+            "Main REML PCG did not converge: "
+            f"relative residual={main_rel:.3e}, tolerance={minq_tol:.3e}, "
+            f"iterations={int(k_pcg)}/{int(maxiter)}."
+        )
+    counters["pcg_iters"] = counters.get("pcg_iters", 0) + int(k_pcg)
+    warm_all_next = sol_all
+
+    HinvXyZ = sol_all[:, : ctx.n_XyZ_cols]
+    chol = None
+    HinvX = None
+    if ctx.xmat is not None and ctx.xmat.shape[1] > 0:
+        xc = ctx.xmat.shape[1]
+        HinvX = HinvXyZ[:, :xc]
+        XtHinvX = (ctx.xmat.double().T @ HinvX.double()).cpu().numpy()
+        chol = _stable_cho_factor_spd(XtHinvX)
+        HinvX64 = HinvX.double()
+        X64 = ctx.xmat.double()
+
+        def proj(mat):
+            rhs = (X64.T @ mat.double()).cpu().numpy()
+            mid = torch.as_tensor(sla.cho_solve(chol, rhs, check_finite=False), device=mat.device)
+            return (mat.double() - HinvX64 @ mid).to(torch.float32)
+    else:
+        def proj(mat):
+            return mat
+
+    PYstar = proj(HinvXyZ[:, ctx.y_col : ctx.rand_stop])
+    Py = PYstar[:, :1].contiguous()
+    yPy = float((ctx.y.double() * PYstar[:, 0].double()).sum())
+    logdet_x = (2.0 * float(np.sum(np.log(np.clip(np.diag(chol[0]), 1e-30, None))))
+                if chol is not None else 0.0)
+    scale = float(ctx.n)
+
+    def loglik(ld):
+        return -0.5 * (yPy + ld + logdet_x) / scale
+
+    if ctx.affine_slq_cache is not None:
+        logdet, trace_pg = _affine_slq_logdet(ctx.affine_slq_cache, theta_g[0], theta_e[0],
+                                              return_derivatives=True)
+    else:
+        logdet, trace_pg = _multi_slq_logdet_and_score(
+            ctx, Hv, pvec, key_slq, nsamples=slq_samples, m=slq_m, use_reference=use_residual_slq,
+            accept_logdet=(None if min_loglik is None else (lambda v: bool(loglik(v) >= min_loglik))),
+        )
+    ll = loglik(logdet)
+    if not math.isfinite(ll):
+        raise FloatingPointError("Non-finite REML objective.")
+    if min_loglik is not None and not ll >= min_loglik:
+        return (ll, None, None, int(k_pcg), warm_all_next, None, None, None, logdet)
+
+    gen = _apply_genetic_stack(ctx, Py)
+    res = _apply_residual_stack(ctx, Py)
+    GPy_stack = torch.cat([gen, res], dim=0)                       # (G+E, n, 1)
+    q_sel = (Py[:, 0].double()[None, :] * GPy_stack[:, :, 0].double()).sum(1).cpu().numpy()
+    trace_pg = np.asarray(trace_pg, dtype=np.float64)
+    if chol is not None:
+        if ctx.affine_slq_cache is not None:
+            KHX = _apply_genetic_stack(ctx, HinvX)[0]
+            rhs_list = [(HinvX.double().T @ KHX.double()), (HinvX.double().T @ HinvX.double())]
+        else:
+            stack = _apply_genetic_stack(ctx, HinvX)
+            rhs_list = [HinvX.double().T @ stack[g].double() for g in range(ctx.G)]
+            for e in range(ctx.E):
+                prod = HinvX if ctx.residual_diag_stack is None else ctx.residual_diag_stack[e, :, None] * HinvX
+                rhs_list.append(HinvX.double().T @ prod.double())
+        correction = np.array([np.trace(sla.cho_solve(chol, r.cpu().numpy(), check_finite=False))
+                               for r in rhs_list])
+        trace_pg = trace_pg - correction
+
+    stats = FisherSolveStats(free_dim=int(ctx.G + ctx.E))
+    GPy_cols = GPy_stack[:, :, 0].T.contiguous()                   # (n, G+E)
+    if warm_ai_ready and warm_ai is not None and warm_ai.shape[1] == ctx.G + ctx.E:
+        X0a = warm_ai
+    elif M_cur is not None:
+        X0a = M_cur(GPy_cols)
+    else:
+        X0a = torch.zeros_like(GPy_cols)
+    t0 = time.perf_counter()
+    HinvGPy, ai_rel, k_ai = pcg_solve(Hv, GPy_cols, M=M_cur, tol=minq_tol, maxiter=maxiter, X0=X0a)
+    del X0a
+    if not math.isfinite(ai_rel) or ai_rel > minq_tol * 1.05:
+        raise FloatingPointError(
+            "Average-information PCG did not converge: "
+            f"relative residual={ai_rel:.3e}, tolerance={minq_tol:.3e}, "
+            f"iterations={int(k_ai)}/{int(maxiter)}."
+        )
+    counters["ai_pcg_iters"] = counters.get("ai_pcg_iters", 0) + int(k_ai)
+    stats.ai_pcg_iters = int(k_ai)
+    stats.ai_elapsed_sec = time.perf_counter() - t0
+    PGPy = proj(HinvGPy)
+    AI_sel = 0.5 * (GPy_cols.double().T @ PGPy.double()).cpu().numpy() / scale
+    AI = 0.5 * (AI_sel + AI_sel.T)
+    FI = AverageInfoMatrix(mat=AI, stats=stats)
+    grad = 0.5 * (q_sel - trace_pg) / scale
+    if not (math.isfinite(ll) and np.all(np.isfinite(grad)) and np.all(np.isfinite(AI))):
+        raise FloatingPointError("Non-finite REML objective, score, or AI matrix.")
+    return (ll, grad, FI, int(k_pcg), warm_all_next, HinvGPy, None, None, logdet)
+
+
+# --------------------------------------------------------------------------
+# Fisher step (host FP64, SciPy) — reml.py:974-1247
+# --------------------------------------------------------------------------
+
+def _freeze_mask(theta_g, grad_g, zero_tol):
+    return np.logical_and(np.asarray(theta_g) <= zero_tol, np.asarray(grad_g) <= 0.0)
+
+
+def _solve_average_info_bound_qp(param, grad, FI, *, lower_step, initial_active, trial_alpha,
+                                 bound_tol):
+    """reml.py:979-1131: bound-constrained Fisher QP via NNLS + KKT polish."""
+    param_h = np.asarray(param, np.float64)
+    grad_h = np.asarray(grad, np.float64)
+    mat_h = np.asarray(FI.mat, np.float64)
+    lower_h = np.asarray(lower_step, np.float64)
+    n_param = int(param_h.size)
+    if mat_h.shape != (n_param, n_param):
+        raise ValueError("Average-information matrix shape does not match the parameter vector.")
+    if not (np.all(np.isfinite(grad_h)) and np.all(np.isfinite(mat_h)) and np.all(np.isfinite(lower_h))):
+        raise FloatingPointError("Non-finite value in the bound-constrained Fisher system.")
+    ridge = float(FI.ridge)
+    if not math.isfinite(ridge) or ridge < 0.0:
+        raise ValueError("Average-information ridge must be finite and nonnegative.")
+    system = 0.5 * (mat_h + mat_h.T) + ridge * np.eye(n_param)
+    scale = float(np.mean(np.abs(np.diag(system))))
+    if not math.isfinite(scale) or scale <= 0.0:
+        scale = 1.0
+    chol = None
+    solved = system
+    for attempt in range(9):
+        jitter = 0.0 if attempt == 0 else 1e-8 * scale * (10.0 ** (attempt - 1))
+        cand = system if jitter == 0.0 else system + jitter * np.eye(n_param)
+        try:
+            chol = sla.cholesky(cand, lower=True, check_finite=False)
+            solved = cand
+            break
+        except np.linalg.LinAlgError:
+            continue
+    if chol is None:
+        raise FloatingPointError("Failed to stabilize the bound-constrained Fisher system.")
+    shifted_rhs = grad_h - solved @ lower_h
+    target = sla.solve_triangular(chol, shifted_rhs, lower=True, check_finite=False)
+    shifted_step, _ = nnls(chol.T, target, maxiter=max(3 * n_param, 1))
+    param_tol = max(float(bound_tol), 1e-10)
+    active = float(trial_alpha) * shifted_step <= param_tol
+    shifted_step[active] = 0.0
+    kkt_scale = max(1.0, float(np.max(np.abs(grad_h), initial=0.0)))
+    kkt_tol = max(1e-7, 1e-6 * kkt_scale)
+    for _ in range(max(5 * n_param, 1)):
+        free = ~active
+        shifted_step.fill(0.0)
+        if np.any(free):
+            try:
+                fc = sla.cho_factor(solved[np.ix_(free, free)], lower=True, check_finite=False)
+                shifted_step[free] = sla.cho_solve(fc, shifted_rhs[free], check_finite=False)
+            except np.linalg.LinAlgError as err:
+                raise FloatingPointError("Failed to polish the NNLS Fisher active set.") from err
+            nonpos = free & (shifted_step <= 0.0)
+            if np.any(nonpos):
+                active[nonpos] = True
+                continue
+        kkt_res = shifted_rhs - solved @ shifted_step
+        viol = active & (kkt_res > kkt_tol)
+        if np.any(viol):
+            cand = np.flatnonzero(viol)
+            active[int(cand[np.argmax(kkt_res[cand])])] = False
+            continue
+        break
+    else:
+        raise FloatingPointError("NNLS Fisher active-set polish did not converge.")
+    step = lower_h + shifted_step
+    kkt = grad_h - solved @ step
+    free_err = float(np.max(np.abs(kkt[~active]), initial=0.0))
+    act_err = float(np.max(kkt[active], initial=0.0))
+    if free_err > kkt_tol or act_err > kkt_tol:
+        raise FloatingPointError(
+            "NNLS Fisher solve did not satisfy KKT tolerance: "
+            f"free_error={free_err:.3e}, active_error={act_err:.3e}, tolerance={kkt_tol:.3e}."
+        )
+    unconstrained = sla.cho_solve((chol, True), grad_h, check_finite=False)
+    would_block = unconstrained < lower_h - param_tol
+    initial_active = np.asarray(initial_active, bool).reshape(-1)
+    provisional = initial_active | would_block
+    fixed_total = int(np.count_nonzero(active & ~initial_active))
+    released_total = int(np.count_nonzero(provisional & ~active))
+    return step, active, fixed_total, released_total
+
+
+def _projected_fisher_direction(param, grad, FI, *, n_genetic=None, genetic_zero_tol,
+                                residual_floor=0.0, trial_alpha=1.0, workset_log_fn=None):
+    """reml.py:1134-1221."""
+    param = np.asarray(param, np.float64)
+    grad = np.asarray(grad, np.float64)
+    G = int(param.size - 1) if n_genetic is None else int(n_genetic)
+    if G < 0 or G > param.size:
+        raise ValueError("n_genetic must be between 0 and len(param).")
+    if residual_floor < 0.0:
+        raise ValueError("residual_floor must be >= 0.")
+    if trial_alpha <= 0.0:
+        raise ValueError("trial_alpha must be > 0.")
+    n_param = int(param.size)
+    active = np.zeros(n_param, bool)
+    active[:G] = _freeze_mask(param[:G], grad[:G], genetic_zero_tol)
+    if G < n_param:
+        active[G:] = (param[G:] <= residual_floor + genetic_zero_tol) & (grad[G:] <= 0.0)
+    lower = np.concatenate([np.zeros(G), np.full(n_param - G, residual_floor)])
+    lower_step = (lower - param) / float(trial_alpha)
+    step, final_active, fixed_total, released_total = _solve_average_info_bound_qp(
+        param, grad, FI, lower_step=lower_step, initial_active=active, trial_alpha=trial_alpha,
+        bound_tol=genetic_zero_tol)
+    free_dim = int(np.count_nonzero(~final_active))
+    frozen = int(np.count_nonzero(final_active[:G]))
+    if FI.stats is not None:
+        FI.stats.free_dim = free_dim
+        FI.stats.frozen_genetic = frozen
+        FI.stats.ws_resolve_count = 1
+        FI.stats.ws_fixed_total = fixed_total
+        FI.stats.ws_released_total = released_total
+        FI.stats.ws_trace = (f"solver=nnls free={free_dim} freeze={frozen} "
+                             f"add={fixed_total} drop={released_total}")
+    if workset_log_fn is not None:
+        workset_log_fn({"resolve_idx": 1, "free_dim": free_dim, "frozen_genetic": frozen,
+                        "fixed_this_resolve": fixed_total, "released_this_resolve": released_total})
+    return step, final_active[:G].copy()
+
+
+def _apply_projected_step(param, step_dir, alpha, *, n_genetic=None, genetic_zero_tol,
+                          residual_floor):
+    """reml.py:1224-1247."""
+    param = np.asarray(param, np.float64)
+    step_dir = np.asarray(step_dir, np.float64)
+    G = int(param.size - 1) if n_genetic is None else int(n_genetic)
+    tg = np.maximum(param[:G] + alpha * step_dir[:G], 0.0)
+    tg = np.where(tg <= genetic_zero_tol, 0.0, tg)
+    te = np.maximum(param[G:] + alpha * step_dir[G:], residual_floor)
+    new = np.concatenate([tg, te])
+    return new, new - param
+
+
+# --------------------------------------------------------------------------
+# fit_reml — reml.py:1254-2092
+# --------------------------------------------------------------------------
+
+def _probe_seeds(seed: int):
+    """Fixed probe seeds per fit (reference: split(PRNGKey(seed + 2026), 3))."""
+    base = int(seed) + 2026
+    return base * 3 + 1, base * 3 + 2
+
+
+def fit_reml(
+    y, K_mvs: Sequence[Callable], diag_list, covar, n_rand_vec: int, maxiter: int,
+    seed: int = 0, h2_init: float = 0.5, param_init=None, minq_iter: int = 50,
+    slq_samples: int = 30, slq_m: int = 30, slq_mode: str = "raw",
+    precond_conf: Optional[ProjectedCorePrecondConf] = None,
+    slq_precond_conf: Optional[ProjectedCorePrecondConf] = None,
+    precond_refresh_fn=None, precond_eps: float = 1e-6, weighted_hv=None, stacked_kv=None,
+    residual_diag_list=None, residual_floor: float = 1e-2, genetic_zero_tol: float = 1e-8,
+    max_linesearch_trials: int = 3, optimizer: str = "strict", scoring_step_tol: float = 1e-4,
+    rel_dll_tol: float = 1e-3, taylor_threshold: float = 0.01, pcg_tol: float = 1e-3,
+    response_is_standardized: bool = False, unit_variance_components: bool = False,
+    verbose: bool = True, log_detail: str = "full", return_diagnostics: bool = False,
+    probe_cache: REMLProbeCache | None = None, mean_information=None,
+    slq_workspace_bytes: int = 256 * 1024**2, counters: dict | None = None,
+):
+    """Fit single-trait Gaussian REML with AI/Fisher updates (reml.py:1254-1311).
+
+    Returns ``(theta, history)`` or ``(theta, history, diagnostics)``;
+    theta is a float64 NumPy vector ``[theta_g..., theta_e...]``.
+    """
+    dev = _dev()
+    y = torch.as_tensor(np.asarray(_as_host(y), np.float32).reshape(-1), device=dev)
+    n = int(y.shape[0])
+    if n <= 0:
+        raise ValueError("fit_reml requires at least one phenotype sample.")
+    K_mvs = tuple(K_mvs)
+    diag_list = tuple(diag_list)
+    G = len(K_mvs)
+    if G != len(diag_list):
+        raise ValueError("Length of K_mvs and diag_list must match.")
+    if G == 0:
+        raise ValueError("fit_reml requires at least one genetic covariance component.")
+    if n_rand_vec <= 0:
+        raise ValueError("fit_reml requires n_rand_vec > 0.")
+    if maxiter <= 0:
+        raise ValueError("fit_reml requires maxiter > 0.")
+    if minq_iter < 0:
+        raise ValueError("fit_reml requires minq_iter >= 0.")
+    if slq_m <= 0:
+        raise ValueError("fit_reml requires slq_m > 0.")
+    if slq_workspace_bytes < 1:
+        raise ValueError("fit_reml requires slq_workspace_bytes > 0.")
+    if not 0.0 <= h2_init <= 1.0:
+        raise ValueError("h2_init must lie in [0, 1].")
+    genetic_trace_atoms = (np.ones(G) if unit_variance_components
+                           else _mean_diag_atoms(diag_list, n_samples=n))
+    if residual_diag_list is None:
+        residual_diag_stack = None
+        residual_diag_atoms = np.ones(1)
+        E = 1
+    else:
+        residual_diag_list = tuple(residual_diag_list)
+        if not residual_diag_list:
+            raise ValueError("residual_diag_list must be non-empty when provided.")
+        residual_diag_stack = torch.stack([torch.as_tensor(_as_host(d), dtype=torch.float32, device=dev).reshape(-1)
+                                           for d in residual_diag_list])
+        if int(residual_diag_stack.shape[1]) != n:
+            raise ValueError("residual_diag_list entries must have length matching the number of samples.")
+        if not bool(torch.isfinite(residual_diag_stack).all()):
+            raise ValueError("residual_diag_list contains non-finite values.")
+        if bool((residual_diag_stack < 0).any()):
+            raise ValueError("residual_diag_list entries must be nonnegative.")
+        if bool((residual_diag_stack.sum(0) <= 0).any()):
+            raise ValueError("residual_diag_list must provide positive residual support for every sample.")
+        E = int(residual_diag_stack.shape[0])
+        residual_diag_atoms = (np.ones(E) if unit_variance_components
+                               else residual_diag_stack.double().mean(1).cpu().numpy())
+    if slq_samples <= 0:
+        raise ValueError("fit_reml requires slq_samples > 0.")
+    if slq_mode not in {"raw", "projected_core_residual"}:
+        raise ValueError(f"Unsupported slq_mode={slq_mode!r}. Expected 'raw' or 'projected_core_residual'.")
+    if residual_floor <= 0.0:
+        raise ValueError("residual_floor must be > 0.")
+    if genetic_zero_tol < 0.0:
+        raise ValueError("genetic_zero_tol must be >= 0.")
+    if max_linesearch_trials < 1:
+        raise ValueError("max_linesearch_trials must be >= 1.")
+    if log_detail not in {"compact", "full"}:
+        raise ValueError("log_detail must be 'compact' or 'full'.")
+    if optimizer not in {"strict", "smile_scoring"}:
+        raise ValueError("optimizer must be 'strict' or 'smile_scoring'.")
+    if optimizer != "strict":
+        raise NotImplementedError("Only the strict optimiser is implemented (SMILE scoring is out of scope).")
+    if mean_information is not None:
+        raise NotImplementedError("Sparse mean information is out of scope for this engine.")
+    if scoring_step_tol < 0.0 or rel_dll_tol < 0.0 or taylor_threshold < 0.0:
+        raise ValueError("tolerances must be >= 0.")
+    if pcg_tol <= 0.0:
+        raise ValueError("pcg_tol must be > 0.")
+    if G > 1 or residual_diag_stack is not None:
+        pcg_tol = min(pcg_tol, 1e-5)                     # reml.py:1400-1408
+    t_start = time.time()
+    full_log = bool(verbose) and log_detail == "full"
+    if verbose:
+        logger.info("[REML] start @ %s n=%d G=%d E=%d slq_samples=%d slq_mode=%s",
+                    datetime.now().isoformat(timespec="seconds"), n, G, E, slq_samples, slq_mode)
+    _, key_slq_fixed = _probe_seeds(seed)
+
+    if response_is_standardized:
+        if not bool(torch.isfinite(y).all()):
+            raise ValueError("Phenotype contains non-finite values.")
+        y_mean = torch.tensor(0.0)
+        y_scale = torch.tensor(1.0)
+    else:
+        y, y_mean, y_scale = standardize_response(y)
+    y_mean_h, y_scale_h = float(y_mean), float(y_scale)
+
+    xmat = None
+    if covar is not None:
+        xmat = torch.as_tensor(np.asarray(_as_host(covar), np.float32), device=dev)
+        if xmat.numel() == 0:
+            xmat = None
+    if xmat is not None:
+        if xmat.ndim == 1:
+            xmat = xmat[:, None]
+        if xmat.ndim != 2 or int(xmat.shape[0]) != n:
+            raise ValueError("covar must be a 2D matrix with one row per phenotype sample.")
+        if not bool(torch.isfinite(xmat).all()):
+            raise ValueError("covar contains non-finite values.")
+        if int(xmat.shape[1]) >= n:
+            raise ValueError("covar must have fewer columns than samples for REML.")
+        _validate_fixed_effect_design(xmat)
+        xmat = xmat.contiguous()
+
+    if param_init is not None:
+        p0 = np.asarray(_as_host(param_init), np.float64).reshape(-1)
+        if p0.size != G + E:
+            raise ValueError(f"param_init length mismatch: expected {G + E}, got {p0.size}")
+        if not np.all(np.isfinite(p0)):
+            raise ValueError("param_init contains non-finite values.")
+        tg0 = np.maximum(p0[:G], 0.0)
+        tg0 = np.where(tg0 <= genetic_zero_tol, 0.0, tg0)
+        param = np.concatenate([tg0, np.maximum(p0[G:], residual_floor)])
+    else:
+        gsum = float(np.sum(genetic_trace_atoms))
+        if not math.isfinite(gsum) or gsum <= 0.0:
+            raise ValueError("At least one genetic component must have positive average diagonal.")
+        rsum = float(np.sum(residual_diag_atoms))
+        if not math.isfinite(rsum) or rsum <= 0.0:
+            raise ValueError("Residual components must have positive total average diagonal.")
+        tg = np.where(genetic_trace_atoms > 0.0, h2_init / gsum, 0.0)
+        te = np.full(E, (1.0 - h2_init) / rsum)
+        param = np.concatenate([np.maximum(tg, 0.0), np.maximum(te, residual_floor)])
+
+    diag_atoms = None
+    diag_stack = None
+    if precond_conf is None or getattr(precond_conf, "diag_mode", None) != "scalar_identity":
+        diag_atoms = _scalar_diag_from_diag_list(diag_list)
+        if diag_atoms is None:
+            diag_stack = torch.stack([
+                torch.full((n,), float(np.asarray(_as_host(d))), device=dev)
+                if np.asarray(_as_host(d)).ndim == 0
+                else torch.as_tensor(_as_host(d), dtype=torch.float32, device=dev) for d in diag_list])
+
+    cache_sig = (K_mvs, n, int(seed), 0, int(slq_samples), int(slq_m), residual_diag_stack is None, str(dev))
+    cache_hit = probe_cache is not None and probe_cache.signature == cache_sig
+    affine = probe_cache.affine_slq if cache_hit else None
+    if G == 1 and E == 1 and residual_diag_stack is None and affine is None:
+        t0 = time.time()
+        affine = _build_affine_slq_cache(K_mvs[0], n, key_slq_fixed, nsamples=slq_samples, m=slq_m)
+        if full_log:
+            logger.info("[REML] affine single-GRM SLQ done elapsed=%.1fs", time.time() - t0)
+    if probe_cache is not None:
+        probe_cache.signature = cache_sig
+        probe_cache.affine_slq = affine
+
+    if slq_precond_conf is None:
+        slq_precond_conf = precond_conf
+    slq_ref = None
+    if (slq_mode == "projected_core_residual" and affine is None and slq_precond_conf is not None
+            and getattr(slq_precond_conf, "diag_mode", None) == "scalar_identity"
+            and getattr(slq_precond_conf, "total_rank", 0) > 0):
+        d_ref = scalar_diag_from_precond_conf(slq_precond_conf, param[:G], param[G:])
+        if d_ref is None:
+            raise ValueError("projected_core_residual SLQ requires a scalar-identity reference.")
+        slq_ref = build_projected_core_runtime(slq_precond_conf, param[:G], d_ref, eps=precond_eps,
+                                               need_invsqrt=True)
+        if slq_ref is None:
+            raise RuntimeError("Failed to build the fixed projected-core SLQ reference.")
+
+    parts = []
+    x_cols = 0
+    if xmat is not None and xmat.shape[1] > 0:
+        x_cols = int(xmat.shape[1])
+        parts.append(xmat)
+    y_col = x_cols
+    parts.append(y[:, None])
+    rand_stop = x_cols + 1
+    rhs_const = torch.cat(parts, dim=1).contiguous()
+    ctx = REMLContext(
+        n=n, G=G, E=E, K_mvs=K_mvs, weighted_hv=weighted_hv, stacked_kv=stacked_kv,
+        diag_stack=diag_stack, residual_diag_stack=residual_diag_stack, xmat=xmat, y=y,
+        rhs_const=rhs_const, y_col=y_col, rand_stop=rand_stop, n_XyZ_cols=rand_stop, R_rand=0,
+        precond_conf=precond_conf, diag_atoms=diag_atoms, residual_diag_atoms=residual_diag_atoms,
+        affine_slq_cache=affine, slq_reference_runtime=slq_ref,
+        slq_workspace_bytes=int(slq_workspace_bytes), counters=counters,
+    )
+    warm_all = torch.full_like(rhs_const, float("nan"))
+    warm_ai = torch.full((n, G + E), float("nan"), device=dev)
+    warm_ready = warm_ai_ready = False
+
+    def run_eval(pvec, warm, warm_ai_cur, wr, war, *, min_loglik=None):
+        return _eval_once(ctx, pvec, warm, warm_ai=warm_ai_cur, key_slq=key_slq_fixed,
+                          minq_tol=pcg_tol, maxiter=maxiter, precond_eps=precond_eps,
+                          slq_samples=slq_samples, slq_m=slq_m, slq_mode=slq_mode,
+                          warm_ready=wr, warm_ai_ready=war, min_loglik=min_loglik)
+
+    t0 = time.time()
+    ll, grad, FI, k_pcg0, warm_all, warm_ai, _, _, logdet_cached = run_eval(
+        param, warm_all, warm_ai, warm_ready, warm_ai_ready)
+    warm_ready = warm_ai_ready = True
+    if not (math.isfinite(ll) and np.all(np.isfinite(grad)) and np.all(np.isfinite(FI.mat))):
+        raise FloatingPointError("Non-finite warmup state (ll/grad/FI).")
+    if full_log:
+        logger.info("[REML] warmup done elapsed=%.1fs pcg=%d ai_pcg=%d", time.time() - t0,
+                    int(k_pcg0), int(FI.stats.ai_pcg_iters))
+
+    stop_reason = "max_iter"
+    history = []
+    for it in range(minq_iter):
+        iter_t0 = time.time()
+        refreshed = False
+        trial_count = 0
+        accepted = False
+        alpha_try = 1.0
+        alpha_used = alpha_try
+        ls_trace = []
+        evaluated = 0
+        last_err = None
+        ll_new, grad_new, FI_new = ll, grad, FI
+        warm_next, warm_ai_next = warm_all, warm_ai
+        eval_elapsed = 0.0
+        step_elapsed = 0.0
+        k_pcg = 0
+        eval_ai_pcg = 0
+        max_rel_dp = 0.0
+        step_norm = 0.0
+        delta_param = np.zeros_like(param)
+        param_updated = param
+        freeze_mask = np.zeros(G, bool)
+        fisher_dict = FisherSolveStats(free_dim=int(param.size)).to_dict()
+        while trial_count < max_linesearch_trials:
+            trial_count += 1
+            alpha_used = alpha_try
+            st0 = time.perf_counter()
+            step_dir, freeze_mask = _projected_fisher_direction(
+                param, grad, FI, n_genetic=G, genetic_zero_tol=genetic_zero_tol,
+                residual_floor=residual_floor, trial_alpha=alpha_used)
+            step_elapsed += time.perf_counter() - st0
+            if FI.stats is not None:
+                fisher_dict = FI.stats.to_dict()
+            param_updated, delta_param = _apply_projected_step(
+                param, step_dir, alpha_used, n_genetic=G, genetic_zero_tol=genetic_zero_tol,
+                residual_floor=residual_floor)
+            pscale = np.maximum(np.abs(param), 1e-8)
+            step_norm = float(np.linalg.norm(delta_param))
+            max_rel_dp = float(np.max(np.abs(delta_param) / pscale))
+            et0 = time.time()
+            try:
+                ll_try, grad_try, FI_try, k_try, warm_try, warm_ai_try, _, _, logdet_try = run_eval(
+                    param_updated, warm_all, warm_ai, warm_ready, warm_ai_ready, min_loglik=ll)
+            except FloatingPointError as err:
+                eval_elapsed += time.time() - et0
+                last_err = err
+                ls_trace.append((alpha_used, float("-inf"), False, 0, 0, False))
+                alpha_try *= 0.5
+                continue
+            eval_elapsed += time.time() - et0
+            dll = float(ll_try - ll)
+            evaluated += 1
+            ai_trial = int(FI_try.stats.ai_pcg_iters) if FI_try is not None and FI_try.stats else 0
+            eval_ai_pcg = ai_trial
+            ok = dll >= 0.0
+            ls_trace.append((alpha_used, dll, ok, int(k_try), ai_trial, False))
+            if ok:
+                accepted = True
+                ll_new, grad_new, FI_new = ll_try, grad_try, FI_try
+                warm_next, warm_ai_next = warm_try, warm_ai_try
+                logdet_new = logdet_try
+                k_pcg = int(k_try)
+                break
+            alpha_try *= 0.5
+        if evaluated == 0:
+            raise FloatingPointError(
+                f"REML line search could not evaluate any of {trial_count} candidates at iteration "
+                f"{it + 1}: {last_err}") from last_err
+        dll = float(ll_new - ll)
+        rel = dll / max(abs(float(ll)), 1e-12)
+        if accepted:
+            returned = param_updated
+            status = "accept" if dll >= 0.0 else "accept_downhill"
+            should_stop = (not math.isfinite(rel)) or rel < rel_dll_tol
+        else:
+            returned = param
+            status = "ll_down"
+            should_stop = True
+        if accepted and not should_stop and (it + 1) < minq_iter and precond_refresh_fn is not None:
+            new_conf = precond_refresh_fn(param_updated)
+            if new_conf is None:
+                raise RuntimeError("PCG preconditioner refresh returned no configuration.")
+            ctx.precond_conf = new_conf
+            refreshed = True
+        history.append({
+            "iter": it + 1, "status": status, "accepted": accepted, "converged": bool(should_stop),
+            "grad_norm": float(np.linalg.norm(grad_new)), "loglik": float(ll_new),
+            "loglik_prev": float(ll), "dll_true": dll, "rel_dll": rel, "pcg_iters": k_pcg,
+            "pcg_tol": pcg_tol, "step_norm": step_norm, "step_sec": step_elapsed,
+            "step_alpha": alpha_used, "line_search_trials": trial_count,
+            "line_search_trace": [
+                {"alpha": float(a), "dll": float(d), "accepted": bool(o), "pcg_iters": int(p),
+                 "ai_pcg_iters": int(q), "used_taylor": bool(t)} for a, d, o, p, q, t in ls_trace],
+            "n_freeze": int(np.sum(freeze_mask)), "eval_ai_pcg_iters": int(eval_ai_pcg),
+            "eval_sec": eval_elapsed, "iter_sec": time.time() - iter_t0,
+            "params": [float(v) for v in np.asarray(returned).reshape(-1)],
+            "slq_taylor": False, "max_rel_dp": max_rel_dp, "pcg_precond_refreshed": refreshed,
+            "y_mean": y_mean_h, "y_scale": y_scale_h, **fisher_dict,
+        })
+        if full_log:
+            logger.info("[REML] iter %d/%d %s ll=%.6e dll=%+.3e alpha=%.3e ls=%d pcg=%d ai_pcg=%d "
+                        "params=%s", it + 1, minq_iter, status, float(ll_new), dll, alpha_used,
+                        trial_count, k_pcg, eval_ai_pcg, np.array2string(np.asarray(returned), precision=5))
+        if accepted:
+            param, grad, FI, ll = param_updated, grad_new, FI_new, ll_new
+            warm_all, warm_ai = warm_next, warm_ai_next
+            logdet_cached = logdet_new
+            if should_stop:
+                stop_reason = "rel_dll"
+                break
+        else:
+            stop_reason = "ll_down"
+            break
+    if history:
+        history[-1]["stop_reason"] = stop_reason
+    if verbose:
+        logger.info("[REML] done elapsed=%.1fs stop=%s", time.time() - t_start, stop_reason)
+    if return_diagnostics:
+        diag = {"theta": param, "grad": grad, "ai": FI.mat, "loglik": ll, "stop_reason": stop_reason,
+                "y_mean": y_mean_h, "y_scale": y_scale_h, "logdet": logdet_cached}
+        if not unit_variance_components:
+            diag.update({"genetic_trace_atoms": genetic_trace_atoms,
+                         "residual_trace_atoms": residual_diag_atoms})
+        return param, history, diag
+    return param, history
+
+
+__all__ = ["fit_reml", "standardize_response"]

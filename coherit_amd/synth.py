@@ -1,0 +1,132 @@
+"""Synthetic workloads: genotypes generated straight into HBM.
+
+Genotypes: per-SNP minor-allele frequency p ~ U(0.05, 0.5); g ~ Binomial(2, p)
+under HWE via integer thresholds of a Philox4x32-10 stream (the
+``coh_panel_generate`` kernel); 1% missing at random.  Phenotypes: intercept
++ N(0,1) covariates with N(0,1) effects + a genetic term on 1% causal SNPs
+with N(0,1) effects on the standardized genotypes, scaled to the target
+per-component h2, + Gaussian noise.
+"""
+from __future__ import annotations
+
+import dataclasses
+
+import numpy as np
+import torch
+
+TWO32 = 4294967296.0
+
+
+def maf_thresholds(p, miss_rate):
+    """uint32 thresholds: g = 0 below t0, 1 below t1, else 2; missing below tm."""
+    p = np.asarray(p, dtype=np.float64)
+    t0 = np.minimum(np.floor((1.0 - p) ** 2 * TWO32), TWO32 - 1).astype(np.uint32)
+    t1 = np.minimum(np.floor((1.0 - p * p) * TWO32), TWO32 - 1).astype(np.uint32)
+    tm = int(min(np.floor(float(miss_rate) * TWO32), TWO32 - 1))
+    return t0, t1, tm
+
+
+class SyntheticSource:
+    """A genotype source that is generated on the GPU (no host genotype bytes).
+
+    ``n`` samples, ``m`` SNPs with MAF ``p`` (array) and missing rate
+    ``miss_rate``; SNP j is always drawn from counter stream j, so any cache
+    permutation (component partitions) sees the same genotypes.
+    """
+
+    missing_val = -127
+
+    def __init__(self, n: int, p, seed: int, miss_rate: float = 0.01):
+        self.n = int(n)
+        self.p = np.asarray(p, dtype=np.float64).reshape(-1)
+        self.m = int(self.p.size)
+        self.seed = int(seed)
+        self.miss_rate = float(miss_rate)
+        self.t0, self.t1, self.tmiss = maf_thresholds(self.p, self.miss_rate)
+
+    def generate_into(self, panel, cache_to_source):
+        ids = np.asarray(cache_to_source, dtype=np.int64)
+        panel.generate(self.seed, ids.astype(np.uint32), self.t0[ids], self.t1[ids], self.tmiss)
+
+    def close(self):
+        return None
+
+
+@dataclasses.dataclass
+class Workload:
+    name: str
+    n: int
+    m: int
+    seed: int
+    p: np.ndarray
+    components: list            # list of source-index arrays (one per component)
+    h2: np.ndarray              # per-component h2
+    n_covar: int
+
+
+def workload(cfg: int, *, scale: float = 1.0, seed_base: int = 20261203) -> Workload:
+    """The synthetic workloads (n, m, partition, h2).  ``scale`` shrinks n and m."""
+    rng = np.random.default_rng(seed_base + cfg)
+    sizes = {0: (5000, 20000), 1: (100000, 500000), 2: (200000, 1000000),
+             3: (300000, 600000), 4: (400000, 400000)}
+    n, m = sizes[cfg]
+    n = max(64, int(round(n * scale)))
+    m = max(64, int(round(m * scale)))
+    p = rng.uniform(0.05, 0.5, size=m)
+    if cfg == 0:
+        comps = [np.arange(m)]
+        h2 = np.array([0.5])
+        ncov = 2
+    elif cfg in (1, 3):
+        comps = [np.flatnonzero(p < 0.1), np.flatnonzero(p >= 0.1)]
+        h2 = np.array([0.15, 0.35])
+        ncov = 10
+    elif cfg == 2:
+        frac = rng.dirichlet(np.ones(8))
+        sizes8 = np.maximum(1, np.floor(frac * m).astype(np.int64))
+        sizes8[-1] = m - sizes8[:-1].sum()
+        edges = np.concatenate([[0], np.cumsum(sizes8)])
+        comps = [np.arange(edges[g], edges[g + 1]) for g in range(8)]
+        h2 = rng.dirichlet(np.full(8, 2.0)) * 0.6
+        ncov = 10
+    else:
+        comps = [np.arange(m)]
+        h2 = np.array([0.4])
+        ncov = 10
+    return Workload(f"cfg{cfg}", n, m, seed_base + cfg, p, comps, h2, ncov)
+
+
+def phenotype(streamer, wl: Workload, *, seed: int | None = None):
+    """(y, X) on the host: X = [1, N(0,1) covariates]; y = X beta + genetic + noise.
+
+    The genetic term of component g is Z_g b_g with 1% causal SNPs
+    (b ~ N(0,1)) computed on the GPU by one Z.b pass per component, scaled
+    to variance h2_g; noise has variance 1 - sum h2.
+    """
+    rng = np.random.default_rng(wl.seed + 7 if seed is None else seed)
+    n = streamer.n
+    X = np.concatenate([np.ones((n, 1)), rng.standard_normal((n, wl.n_covar))], axis=1)
+    beta = rng.standard_normal(wl.n_covar + 1)
+    off = streamer._component_snp_offsets
+    gen = np.zeros(n)
+    for g in range(streamer.n_components):
+        a, b = int(off[g]), int(off[g + 1])
+        size = b - a
+        k = max(1, int(round(0.01 * size)))
+        causal = rng.choice(size, size=k, replace=False) + a
+        bvec = np.zeros((streamer.m, 1), np.float32)
+        bvec[causal, 0] = rng.standard_normal(k)
+        zb = zb_product(streamer, bvec)
+        sd = float(np.std(zb))
+        if sd > 0:
+            gen += np.sqrt(wl.h2[g]) * (zb - zb.mean()) / sd
+    noise = rng.standard_normal(n) * np.sqrt(max(1e-6, 1.0 - float(np.sum(wl.h2))))
+    y = X @ beta + gen + noise
+    return y.astype(np.float32), X.astype(np.float32)
+
+
+def zb_product(streamer, b) -> np.ndarray:
+    """Z b (n,) for cache-order SNP effects b (m, 1), computed on the GPU."""
+    out = streamer.panel.zb(torch.as_tensor(np.asarray(b, np.float32)), 0,
+                            streamer.n_components, precision=0)
+    return out[:, 0].double().cpu().numpy()

@@ -1,0 +1,203 @@
+/*
+ * coherit_amd — C-ABI of the MI355X (gfx950) matrix-free REML engine.
+ *
+ * Plain pointers and sizes only.  Every array argument named `dev_*` or
+ * documented as "device" is device memory owned by the caller; `host_*`
+ * arrays are host memory.  Streams are `hipStream_t` passed as `void*`.
+ * Every function returns a status (COH_OK == 0); on failure the message is
+ * available from coh_last_error() (thread-local).  The caller-facing Python
+ * layer maps statuses to the reference's exception types:
+ *   COH_ERR_VALUE   -> ValueError         (kv_impl.py:538-571, reml.py:1313-1394)
+ *   COH_ERR_INDEX   -> IndexError         (geno_stream.py:2693-2697)
+ *   COH_ERR_FLOAT   -> FloatingPointError (pcg.py:119,152; reml.py:771-776)
+ *   COH_ERR_RUNTIME -> RuntimeError       (geno_stream.py:2455-2458, 2610-2614)
+ *   COH_ERR_HIP     -> RuntimeError       (HIP runtime failure)
+ *
+ * The panel handle owns the resident genotype panel and its statistics; all
+ * operator outputs and workspaces are caller memory (workspace sizes come
+ * from coh_workspace_bytes).  Distinct handles and streams are re-entrant;
+ * one handle may serve concurrent read-only operator launches.
+ */
+#ifndef COHERIT_AMD_H
+#define COHERIT_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define COH_OK 0
+#define COH_ERR_VALUE 1
+#define COH_ERR_FLOAT 2
+#define COH_ERR_RUNTIME 3
+#define COH_ERR_HIP 4
+#define COH_ERR_INDEX 5
+
+/* Contraction precisions of the operator kernels. */
+#define COH_PREC_FP32 0  /* plain: centred FP32 values, FP32 accumulation   */
+#define COH_PREC_I8X3 1  /* exact int8 limbs (24-bit fixed point) on MFMA   */
+#define COH_PREC_I8X2 2  /* int8 limbs, 16-bit fixed point, on MFMA         */
+
+/* Operation ids for coh_workspace_bytes. */
+#define COH_OP_KV 0
+#define COH_OP_XTV 1
+
+typedef struct coh_panel coh_panel;
+
+/* Last error message of the calling thread ("" when none). */
+const char* coh_last_error(void);
+/* Library ABI version. */
+int coh_version(void);
+
+/* ---------------------------------------------------------------------
+ * Resident panel.  Replaces GenoBlockStreamer's packed tmpfile cache and
+ * pinned H2D ring (geno_stream.py:1167-1453, 148-327): the packed panel is
+ * uploaded once and stays in HBM.
+ * ------------------------------------------------------------------- */
+
+/* Create a panel for n samples from the reference call plan
+ * (geno_stream.py:469-505 `_build_call_geometry`): per call its cache SNP
+ * start, true width and component id.  Calls of one component must be
+ * contiguous and ordered.  Allocates the panel on the current device. */
+int coh_panel_create(int64_t n, int32_t n_calls, const int64_t* host_call_snp_starts,
+                     const int32_t* host_call_true_widths,
+                     const int32_t* host_call_component_ids, int32_t n_components,
+                     coh_panel** out);
+int coh_panel_destroy(coh_panel* panel);
+
+/* info[0..7] = n, m, n_calls, n_components, m_slot, sample-major row bytes,
+ * variant-major row bytes, finalized. */
+int coh_panel_info(const coh_panel* panel, int64_t* host_info);
+/* Per component [begin, end) slot range; host arrays of n_components. */
+int coh_panel_component_slots(const coh_panel* panel, int64_t* host_begin, int64_t* host_end);
+/* First slot of every call; host array of n_calls. */
+int coh_panel_call_slots(const coh_panel* panel, int64_t* host_call_slot);
+
+/* Upload one packed call block in the reference cache layout: (n, ceil(w/4))
+ * bytes, SNP 4q+t of a sample at bits 2t of byte q (geno_stream.py:885-904,
+ * 1326-1331).  Device memory, row stride ld bytes. */
+int coh_panel_ingest_packed_call(coh_panel* panel, int32_t call_idx, const uint8_t* dev_block,
+                                 int64_t ld, void* stream);
+/* Ingest `count` cache-order SNP rows of variant-major int8 genotypes
+ * (rows = SNPs, n values each, row stride ld): statistics from the raw
+ * values and packing through the reference LUT (geno_stream.py:958-1007,
+ * 1263-1270).  Replaces `_stats_and_pack_varmaj_numba`. */
+int coh_panel_ingest_i8_varmaj(coh_panel* panel, int64_t cache_snp_start, int64_t count,
+                               const int8_t* dev_block, int64_t ld, int32_t missing_raw,
+                               void* stream);
+/* Ingest `count` cache-order SNPs from raw SNP-major BED rows (the bytes after
+ * the 3-byte magic, bytes_per_snp = ceil(n_full/4) per row) for the kept
+ * samples dev_sample_idx[0..n) (geno_stream.py:748-848
+ * `_stats_and_transcode_raw_bed_numba`, LUT [2,3,1,0]). */
+int coh_panel_ingest_bed(coh_panel* panel, int64_t cache_snp_start, int64_t count,
+                         const uint8_t* dev_bed_rows, int64_t bytes_per_snp,
+                         const int32_t* dev_sample_idx, void* stream);
+/* Generate all m cache SNPs on the device with the project's counter-based
+ * genotype generator (Philox4x32-10; per-SNP thresholds t0/t1 of the MAF,
+ * shared missing threshold); dev arrays have m entries.  No reference
+ * counterpart: the synthetic workloads' data source. */
+int coh_panel_generate(coh_panel* panel, uint64_t seed, const uint32_t* dev_snp_ids,
+                       const uint32_t* dev_t0, const uint32_t* dev_t1, uint32_t tmiss,
+                       void* stream);
+/* Build the second layout, integer statistics and float32 mean/inv_sd
+ * (geno_stream.py:1780-1792, IEEE division and square root).  Optional
+ * device overrides (m entries each) replace mean/inv_sd
+ * (`standardization_override`, geno_stream.py:1313-1322). */
+int coh_panel_finalize(coh_panel* panel, const float* dev_mean_override,
+                       const float* dev_inv_override, void* stream);
+/* Copy statistics in cache order into device arrays of m entries (any may
+ * be NULL). */
+int coh_panel_copy_stats(const coh_panel* panel, float* dev_mean, float* dev_inv,
+                         int64_t* dev_cnt, int64_t* dev_s1, int64_t* dev_s2, void* stream);
+/* Read one call back in the reference cache layout (n, ceil(w/4)). */
+int coh_panel_read_packed_call(const coh_panel* panel, int32_t call_idx, uint8_t* dev_out,
+                               int64_t ld, void* stream);
+
+/* ---------------------------------------------------------------------
+ * Operators.  Replace kv_impl's per-call jitted kernels and drivers
+ * (kv_impl.py:58-218, 268-439, 911-1339).
+ * ------------------------------------------------------------------- */
+
+/* Workspace bytes for an operator call of width k over components
+ * [comp_begin, comp_end). */
+int coh_workspace_bytes(const coh_panel* panel, int32_t op, int32_t precision, int64_t k,
+                        int32_t comp_begin, int32_t comp_end, uint64_t* host_bytes);
+
+/* out(_g) = beta * out(_g) + sum_{g in [comp_begin, comp_end)}
+ *           host_comp_scale[g] * Z_g (Z_g^T V)
+ * V and out are (n, k) row-major FP32 with leading dimensions ldv/ldo.
+ * stacked == 0: one output; stacked == 1: component g writes
+ * out + (g - comp_begin) * out_stride.  host_comp_scale has n_components
+ * entries (e.g. 1/eff_g, theta_g/eff_g).  beta is 0 or 1.
+ * Covers kv (kv_impl.py:268-326), component_kv (1042-1083), stacked
+ * (1136-1187, 969-1039) and weighted H.V (1086-1133, 911-966). */
+int coh_kv(const coh_panel* panel, int32_t precision, const float* dev_V, int64_t k, int64_t ldv,
+           int32_t comp_begin, int32_t comp_end, const float* host_comp_scale, int32_t stacked,
+           float* dev_out, int64_t ldo, int64_t out_stride, float beta, void* dev_workspace,
+           uint64_t workspace_bytes, void* stream);
+
+/* out = scale * Z^T V for components [comp_begin, comp_end)
+ * (kv_impl.py:163-185, 382-439).  slot_space == 0: rows in cache order
+ * (m, k), covering the requested components' SNPs; slot_space == 1: rows in
+ * slot space (m_slot, k) (padding rows zero) for Gram products
+ * (kv_impl.py:188-195, 1287-1339). */
+int coh_xtv(const coh_panel* panel, int32_t precision, const float* dev_V, int64_t k,
+            int64_t ldv, int32_t comp_begin, int32_t comp_end, float scale, int32_t slot_space,
+            float* dev_out, int64_t ldo, void* dev_workspace, uint64_t workspace_bytes,
+            void* stream);
+
+/* out(_g) = beta * out(_g) + Z_g B_g for components [comp_begin, comp_end):
+ * standardized genotypes times SNP effects (kv_impl.py:221-265
+ * `_zxb_one_call_jit`/`_zxb_multi_one_call_jit`).  B rows are the cache SNPs
+ * of the requested components in cache order ((m_sel, k), leading dim ldb). */
+int coh_zb(const coh_panel* panel, int32_t precision, const float* dev_B, int64_t k, int64_t ldb,
+           int32_t comp_begin, int32_t comp_end, int32_t stacked, float* dev_out, int64_t ldo,
+           int64_t out_stride, float beta, void* dev_workspace, uint64_t workspace_bytes,
+           void* stream);
+
+/* Exact per-sample diagonal sum_j z_ij^2 over all SNPs, FP64 accumulation,
+ * divided by `eff` (geno_stream.py:929-955, 2791-2835). */
+int coh_exact_diag(const coh_panel* panel, double eff, float* dev_out, void* stream);
+
+/* ---------------------------------------------------------------------
+ * Solver kernels (pcg.py:31-65, reml.py:284-304, slq.py:49-74).
+ * Matrices are (n, k) row-major FP32; per-column scalars are FP64.
+ * ------------------------------------------------------------------- */
+
+/* Workspace bytes for the column reductions below. */
+int coh_reduce_workspace_bytes(int64_t n, int64_t k, uint64_t* host_bytes);
+/* out[c] = sum_i A[i,c] * B[i,c] in FP64 (B may equal A). */
+int coh_col_dots(int64_t n, int64_t k, const float* dev_A, int64_t lda, const float* dev_B,
+                 int64_t ldb, double* dev_out, void* dev_workspace, uint64_t workspace_bytes,
+                 void* stream);
+/* _pcg_state_update (pcg.py:31-48): denom = colsum(P*AP); alpha from
+ * rs/denom on moving columns; X += P*alpha; R -= AP*alpha; *dev_breakdown
+ * |= any(active & !valid).  dev_active is int32[k]. */
+int coh_pcg_state_update(int64_t n, int64_t k, float* dev_X, int64_t ldx, float* dev_R,
+                         int64_t ldr, const float* dev_P, int64_t ldp, const float* dev_AP,
+                         int64_t ldap, const double* dev_rs, const int32_t* dev_active,
+                         int32_t* dev_breakdown, void* dev_workspace, uint64_t workspace_bytes,
+                         void* stream);
+/* _pcg_direction_update (pcg.py:51-65): rs_new = colsum(R*Z), rnorm =
+ * ||R||, active = rnorm > threshold, beta = rs_new/rs_old on active
+ * columns, P = active ? Z + beta*P : 0; dev_rs (in: rs_old, out: masked
+ * rs_new), dev_rnorm and dev_active are written. */
+int coh_pcg_direction_update(int64_t n, int64_t k, float* dev_P, int64_t ldp, const float* dev_R,
+                             int64_t ldr, const float* dev_Z, int64_t ldz, double* dev_rs,
+                             const double* dev_threshold, double* dev_rnorm,
+                             int32_t* dev_active, void* dev_workspace,
+                             uint64_t workspace_bytes, void* stream);
+/* Affine Lanczos step (reml.py:284-304 `_lanczos_step_jit`):
+ * w = Kv - beta_prev*prev; alpha = colsum(v*w); w -= alpha*v;
+ * beta = ||w||; v_next = w / (beta > 0 ? beta : 1).  v_next may alias prev. */
+int coh_lanczos_step(int64_t n, int64_t k, const float* dev_v, const float* dev_Kv,
+                     const float* dev_prev, const double* dev_beta_prev, float* dev_v_next,
+                     double* dev_alpha, double* dev_beta, void* dev_workspace,
+                     uint64_t workspace_bytes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* COHERIT_AMD_H */

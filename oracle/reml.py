@@ -1,0 +1,369 @@
+"""Oracle: strict AI-REML restated in NumPy FP64 on dense kernels (reml.py).
+
+Test infrastructure only.  Follows reml.py:1254-2092 (fit_reml),
+reml.py:685-971 (_eval_once), reml.py:367-486 (affine SLQ), reml.py:489-540
+(multi SLQ), reml.py:979-1247 (NNLS Fisher step) and precond.py:42-221
+(projected-core whitening / PCG preconditioner given a basis U).
+Kernels are explicit (n, n) FP64 matrices.  Also provides the exact dense
+REML objective (Cholesky log-determinants) as the limit oracle.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import scipy.linalg as sla
+from scipy.optimize import nnls
+
+from . import pcg as opcg
+from . import slq as oslq
+
+EPS32 = float(np.finfo(np.float32).eps)
+
+
+def standardize_response(y):
+    """reml.py:151-163 (FP32 arithmetic, population std, +1e-6)."""
+    y = np.asarray(y, np.float32).reshape(-1)
+    mean = np.float32(y.mean(dtype=np.float64))
+    std = np.float32(y.astype(np.float64).std())
+    scale = np.float32(std + np.float32(1e-6))
+    return ((y - mean) / scale).astype(np.float32), float(mean), float(scale)
+
+
+def probe_seeds(seed):
+    base = int(seed) + 2026
+    return base * 3 + 1, base * 3 + 2
+
+
+def cho_spd(A, base_jitter=1e-6, max_tries=8):
+    """reml.py:240-277."""
+    A = 0.5 * (A + A.T)
+    scale = float(np.mean(np.abs(np.diag(A))))
+    scale = scale if (math.isfinite(scale) and scale > 0) else 1.0
+    for k in range(-1, max_tries):
+        At = A if k < 0 else A + base_jitter * scale * 10.0 ** k * np.eye(A.shape[0])
+        try:
+            c = sla.cho_factor(At, lower=True)
+        except np.linalg.LinAlgError:
+            continue
+        if np.all(np.diag(c[0]) > 0):
+            return c
+    raise FloatingPointError("Failed to stabilize XtHinvX Cholesky with jitter escalation.")
+
+
+def affine_cache(K, key, nsamples, m):
+    """reml.py:367-412: Lanczos on K without re-orthogonalisation."""
+    n = K.shape[0]
+    z = oslq.rademacher_probes(key, n, nsamples).astype(np.float64)
+    v = (z / np.sqrt(n)).T
+    prev = np.zeros_like(v)
+    bp = np.zeros(nsamples)
+    A, B = [], []
+    for i in range(m):
+        w = K @ v - prev * bp
+        a = (v * w).sum(0)
+        w = w - v * a
+        b = np.linalg.norm(w, axis=0)
+        vn = w / np.where(b > 0, b, 1)
+        A.append(a)
+        if i < m - 1:
+            B.append(b)
+        prev, v, bp = v, vn, b
+    return np.stack(A), (np.stack(B) if B else np.empty((0, nsamples)))
+
+
+def affine_logdet(A, B, n, nsamples, tg, te):
+    """reml.py:452-486."""
+    m, S = A.shape
+    T = np.zeros((S, m, m))
+    i = np.arange(m)
+    T[:, i, i] = A.T
+    if m > 1:
+        o = np.arange(m - 1)
+        T[:, o + 1, o] = B.T
+        T[:, o, o + 1] = B.T
+    ev, vec = np.linalg.eigh(T)
+    w = vec[:, 0, :] ** 2
+    raw = tg * ev + te
+    floor = max(EPS32 * m, 1e-6)
+    eh = np.clip(raw, floor, None)
+    act = (raw > floor).astype(float)
+    f = n / nsamples
+    return (f * np.sum(w * np.log(eh)),
+            np.array([f * np.sum(w * act * ev / eh), f * np.sum(w * act / eh)]))
+
+
+class ProjectedCore:
+    """dI + U C U^T for an orthonormal U with cores C_g = U^T K_g U - a_g I
+    (reml_model.py:863-960; precond.py:42-221)."""
+
+    def __init__(self, U, Ks, atoms):
+        self.U = np.asarray(U, np.float64)
+        cores = [self.U.T @ K @ self.U for K in Ks]
+        r = self.U.shape[1]
+        self.cores = [0.5 * (c + c.T) - a * np.eye(r) for c, a in zip(cores, atoms)]
+        self.atoms = np.asarray(atoms, np.float64)
+
+    def runtime(self, tg, te, eps):
+        d = max(float(np.dot(tg, self.atoms) + np.sum(te)) + eps, 1e-8)
+        mid = sum(t * c for t, c in zip(tg, self.cores))
+        mid = 0.5 * (mid + mid.T) + d * np.eye(mid.shape[0])
+        return d, mid
+
+    def apply_inv(self, tg, te, eps):
+        d, mid = self.runtime(tg, te, eps)
+        L = np.linalg.cholesky(mid)
+
+        def f(v):
+            rhs = self.U.T @ v
+            z = sla.cho_solve((L, True), rhs)
+            return v / d + self.U @ (z - rhs / d)
+        return f
+
+    def invsqrt(self, tg, te, eps):
+        d, mid = self.runtime(tg, te, eps)
+        ev, vec = np.linalg.eigh(mid)
+        ev = np.clip(ev, 1e-8, None)
+        L = np.linalg.cholesky(mid)
+        n = self.U.shape[0]
+        r = self.U.shape[1]
+        logdet = (n - r) * np.log(d) + 2 * np.sum(np.log(np.diag(L)))
+
+        def f(v):
+            rhs = self.U.T @ v
+            z = vec @ ((vec.T @ rhs) / np.sqrt(ev)[:, None])
+            return v / np.sqrt(d) + self.U @ (z - rhs / np.sqrt(d))
+        return f, logdet
+
+
+def nnls_step(param, grad, AI, lower_step, trial_alpha, bound_tol, ridge=1e-4):
+    """reml.py:979-1131 (SciPy NNLS + KKT active-set polish)."""
+    npar = param.size
+    system = 0.5 * (AI + AI.T) + ridge * np.eye(npar)
+    scale = float(np.mean(np.abs(np.diag(system)))) or 1.0
+    chol = None
+    for att in range(9):
+        jit = 0.0 if att == 0 else 1e-8 * scale * 10.0 ** (att - 1)
+        cand = system + jit * np.eye(npar)
+        try:
+            chol = sla.cholesky(cand, lower=True)
+            system = cand
+            break
+        except np.linalg.LinAlgError:
+            continue
+    if chol is None:
+        raise FloatingPointError("Failed to stabilize the bound-constrained Fisher system.")
+    srhs = grad - system @ lower_step
+    x, _ = nnls(chol.T, sla.solve_triangular(chol, srhs, lower=True), maxiter=max(3 * npar, 1))
+    ptol = max(bound_tol, 1e-10)
+    active = trial_alpha * x <= ptol
+    x[active] = 0
+    kkt_tol = max(1e-7, 1e-6 * max(1.0, float(np.max(np.abs(grad), initial=0.0))))
+    for _ in range(max(5 * npar, 1)):
+        free = ~active
+        x.fill(0)
+        if free.any():
+            x[free] = sla.cho_solve(sla.cho_factor(system[np.ix_(free, free)], lower=True), srhs[free])
+            bad = free & (x <= 0)
+            if bad.any():
+                active[bad] = True
+                continue
+        res = srhs - system @ x
+        viol = active & (res > kkt_tol)
+        if viol.any():
+            c = np.flatnonzero(viol)
+            active[c[np.argmax(res[c])]] = False
+            continue
+        break
+    return lower_step + x, active
+
+
+class Problem:
+    """Dense operators for one fit: kernels K_g (n, n) with mean-diagonal atoms."""
+
+    def __init__(self, Ks, atoms, residual_diags=None):
+        self.Ks = [np.asarray(K, np.float64) for K in Ks]
+        self.atoms = np.asarray(atoms, np.float64)
+        self.res = None if residual_diags is None else [np.asarray(d, np.float64) for d in residual_diags]
+
+    @property
+    def G(self):
+        return len(self.Ks)
+
+    @property
+    def E(self):
+        return 1 if self.res is None else len(self.res)
+
+    def H(self, p):
+        G = self.G
+        H = sum(p[g] * self.Ks[g] for g in range(G))
+        if self.res is None:
+            return H + p[G] * np.eye(H.shape[0])
+        return H + np.diag(sum(p[G + e] * self.res[e] for e in range(self.E)))
+
+    def derivs(self):
+        n = self.Ks[0].shape[0]
+        out = list(self.Ks)
+        out += [np.eye(n)] if self.res is None else [np.diag(d) for d in self.res]
+        return out
+
+
+def eval_once(pb: Problem, p, y, X, warm, *, key, tol, maxiter, slq_samples, slq_m, slq_mode,
+              core: ProjectedCore | None, ref, eps, min_ll=None, affine=None):
+    """reml.py:685-971 in FP64."""
+    n = y.size
+    G, E = pb.G, pb.E
+    H = pb.H(p)
+    Hv = lambda v: H @ v
+    M = core.apply_inv(p[:G], p[G:], eps) if core is not None else None
+    rhs = np.concatenate([X, y[:, None]], axis=1) if X is not None else y[:, None]
+    X0 = warm if warm is not None else (M(rhs) if M is not None else None)
+    sol, rel, k = opcg.pcg_solve(Hv, rhs, M=M, tol=tol, maxiter=maxiter, X0=X0)
+    if rel > tol * 1.05:
+        raise FloatingPointError("Main REML PCG did not converge")
+    if X is not None:
+        HX = sol[:, : X.shape[1]]
+        chol = cho_spd(X.T @ HX)
+        proj = lambda mat: mat - HX @ sla.cho_solve(chol, X.T @ mat)
+        logdet_x = 2 * np.sum(np.log(np.clip(np.diag(chol[0]), 1e-30, None)))
+    else:
+        chol = None
+        proj = lambda mat: mat
+        logdet_x = 0.0
+    Py = proj(sol[:, -1:])
+    yPy = float(y @ Py[:, 0])
+    ll_of = lambda ld: -0.5 * (yPy + ld + logdet_x) / n
+    D = pb.derivs()
+    if affine is not None:
+        logdet, tr = affine_logdet(affine[0], affine[1], n, slq_samples, p[0], p[1])
+    else:
+        if ref is not None:
+            W, ref_ld = ref
+        else:
+            W, ref_ld = (lambda v: v), 0.0
+
+        def Bv(v):
+            return W(Hv(W(v)))
+
+        def pull(left, right):
+            left, right = W(left), W(right)
+            prods = [Dk @ left for Dk in D]
+            return W(sum(p[i] * prods[i] for i in range(len(D)))), np.array(
+                [float(np.sum(right * pr)) for pr in prods])
+
+        accept = None if min_ll is None else (lambda v: ll_of(v + ref_ld) >= min_ll)
+        logdet, tr = oslq.logdet_value_and_grad(Bv, pull, n, key, nsamples=slq_samples, m=slq_m,
+                                                accept_value=accept)
+        logdet += ref_ld
+    ll = ll_of(logdet)
+    if min_ll is not None and not ll >= min_ll:
+        return ll, None, None, sol, k, logdet
+    GPy = np.stack([Dk @ Py[:, 0] for Dk in D], axis=1)
+    q = Py[:, 0] @ GPy
+    tr = np.asarray(tr, np.float64)
+    if chol is not None:
+        HX = sol[:, : X.shape[1]]
+        if affine is not None:
+            corr = [np.trace(sla.cho_solve(chol, HX.T @ (pb.Ks[0] @ HX))),
+                    np.trace(sla.cho_solve(chol, HX.T @ HX))]
+        else:
+            corr = [np.trace(sla.cho_solve(chol, HX.T @ (Dk @ HX))) for Dk in D]
+        tr = tr - np.asarray(corr)
+    HG, rel2, _ = opcg.pcg_solve(Hv, GPy, M=M, tol=tol, maxiter=maxiter,
+                                 X0=(M(GPy) if M is not None else None))
+    if rel2 > tol * 1.05:
+        raise FloatingPointError("AI PCG did not converge")
+    AI = 0.5 * (GPy.T @ proj(HG)) / n
+    AI = 0.5 * (AI + AI.T)
+    grad = 0.5 * (q - tr) / n
+    return ll, grad, AI, sol, k, logdet
+
+
+def fit_reml(pb: Problem, y, covar, *, seed=0, h2_init=0.5, maxiter=400, minq_iter=50,
+             slq_samples=30, slq_m=30, slq_mode="raw", pcg_tol=1e-3, residual_floor=1e-2,
+             genetic_zero_tol=1e-8, max_linesearch_trials=3, rel_dll_tol=1e-3, precond_U=None,
+             precond_eps=1e-6, response_is_standardized=False, return_history=False):
+    """reml.py:1254-2092 (strict optimiser) in FP64 on dense kernels."""
+    y = np.asarray(y, np.float32).reshape(-1)
+    if not response_is_standardized:
+        y, _, _ = standardize_response(y)
+    y = y.astype(np.float64)
+    n = y.size
+    G, E = pb.G, pb.E
+    if G > 1 or pb.res is not None:
+        pcg_tol = min(pcg_tol, 1e-5)
+    X = None if covar is None else np.asarray(covar, np.float32).astype(np.float64)
+    if X is not None and X.ndim == 1:
+        X = X[:, None]
+    ratoms = np.ones(1) if pb.res is None else np.array([d.mean() for d in pb.res])
+    tg = np.where(pb.atoms > 0, h2_init / pb.atoms.sum(), 0.0)
+    te = np.full(E, (1 - h2_init) / ratoms.sum())
+    p = np.concatenate([np.maximum(tg, 0), np.maximum(te, residual_floor)])
+    _, key = probe_seeds(seed)
+    affine = None
+    if G == 1 and pb.res is None:
+        affine = affine_cache(pb.Ks[0], key, slq_samples, slq_m)
+    core = ProjectedCore(precond_U, pb.Ks, pb.atoms) if precond_U is not None else None
+    ref = None
+    if slq_mode == "projected_core_residual" and affine is None and core is not None:
+        ref = core.invsqrt(p[:G], p[G:], precond_eps)
+    kw = dict(key=key, tol=pcg_tol, maxiter=maxiter, slq_samples=slq_samples, slq_m=slq_m,
+              slq_mode=slq_mode, core=core, ref=ref, eps=precond_eps, affine=affine)
+    ll, grad, AI, warm, _, logdet = eval_once(pb, p, y, X, None, **kw)
+    history = []
+    for it in range(minq_iter):
+        alpha = 1.0
+        accepted = False
+        for _trial in range(max_linesearch_trials):
+            act0 = np.zeros(p.size, bool)
+            act0[:G] = (p[:G] <= genetic_zero_tol) & (grad[:G] <= 0)
+            act0[G:] = (p[G:] <= residual_floor + genetic_zero_tol) & (grad[G:] <= 0)
+            lower = np.concatenate([np.zeros(G), np.full(E, residual_floor)])
+            step, _ = nnls_step(p, grad, AI, (lower - p) / alpha, alpha, genetic_zero_tol)
+            new_g = np.maximum(p[:G] + alpha * step[:G], 0)
+            new_g = np.where(new_g <= genetic_zero_tol, 0, new_g)
+            new = np.concatenate([new_g, np.maximum(p[G:] + alpha * step[G:], residual_floor)])
+            try:
+                ll2, g2, AI2, warm2, _, ld2 = eval_once(pb, new, y, X, warm, min_ll=ll, **kw)
+            except FloatingPointError:
+                alpha *= 0.5
+                continue
+            if ll2 - ll >= 0:
+                accepted = True
+                break
+            alpha *= 0.5
+        if not accepted:
+            history.append({"iter": it + 1, "status": "ll_down", "params": p.tolist()})
+            break
+        rel = (ll2 - ll) / max(abs(ll), 1e-12)
+        p, ll, grad, AI, warm, logdet = new, ll2, g2, AI2, warm2, ld2
+        history.append({"iter": it + 1, "status": "accept", "params": p.tolist(), "loglik": ll})
+        if not math.isfinite(rel) or rel < rel_dll_tol:
+            break
+    if return_history:
+        return p, history, {"loglik": ll, "logdet": logdet}
+    return p
+
+
+def exact_reml_loglik(pb: Problem, p, y, X):
+    """Exact dense restricted log-likelihood in the reference's scaling:
+    -(y'Py + log|H| + log|X'H^-1X|)/(2n)."""
+    H = pb.H(p)
+    L = np.linalg.cholesky(H)
+    logdet = 2 * np.sum(np.log(np.diag(L)))
+    Hi = sla.cho_solve((L, True), np.eye(H.shape[0]))
+    if X is not None:
+        A = X.T @ Hi @ X
+        P = Hi - Hi @ X @ np.linalg.solve(A, X.T @ Hi)
+        ldx = np.linalg.slogdet(A)[1]
+    else:
+        P, ldx = Hi, 0.0
+    return -0.5 * (y @ P @ y + logdet + ldx) / y.size
+
+
+def h2_from_theta(theta, atoms, ratoms=(1.0,)):
+    """reml_model.py:1945-1953."""
+    G = len(atoms)
+    gen = float(np.dot(theta[:G], atoms))
+    res = float(np.dot(theta[G:], ratoms))
+    return gen / max(gen + res, 1e-8)

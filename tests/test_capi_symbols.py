@@ -1,0 +1,50 @@
+"""The C-ABI library loads and exports exactly what include/coherit_amd.h declares."""
+import pathlib
+import re
+import subprocess
+
+import pytest
+
+ROOT = pathlib.Path(__file__).resolve().parents[1]
+HEADER = ROOT / "include" / "coherit_amd.h"
+LIB = ROOT / "coherit_amd" / "_lib" / "libcoherit_amd.so"
+
+
+def declared_symbols():
+    text = HEADER.read_text()
+    return sorted(set(re.findall(r"^(?:int|const char\*)\s+(coh_\w+)\s*\(", text, re.M)))
+
+
+def test_header_declares_entry_points():
+    names = declared_symbols()
+    assert "coh_kv" in names and "coh_panel_create" in names and len(names) >= 20
+
+
+@pytest.mark.skipif(not LIB.exists(), reason="HIP library not built")
+def test_library_exports_every_declared_symbol():
+    out = subprocess.run(["nm", "-D", "--defined-only", str(LIB)], capture_output=True, text=True,
+                         check=True).stdout
+    exported = set(re.findall(r"\sT\s(coh_\w+)$", out, re.M))
+    missing = [s for s in declared_symbols() if s not in exported]
+    assert not missing, f"declared but not exported: {missing}"
+
+
+@pytest.mark.skipif(not LIB.exists(), reason="HIP library not built")
+def test_library_loads_without_gpu_and_binding_covers_header():
+    from coherit_amd import _native
+
+    dll = _native.load_library(LIB)
+    assert dll.coh_version() >= 1
+    assert dll.coh_last_error() == b""
+    assert sorted(_native.SIGNATURES) == declared_symbols()
+
+
+def test_product_path_refuses_to_run_without_gpu(monkeypatch):
+    import torch
+
+    from coherit_amd import _native
+
+    monkeypatch.setattr(torch.cuda, "is_available", lambda: False)
+    monkeypatch.setattr(_native, "_lib", None)
+    with pytest.raises(RuntimeError, match="no CPU fallback"):
+        _native.lib()

@@ -1,0 +1,119 @@
+"""GPU: the operator family matches the FP64 oracle (FP32 plain: <= 1e-5 relative)."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import geno
+from oracle import kv as okv
+from tests.helpers import cache_codes, int8_block, random_codes, rel_err
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"fp32": 1e-5}
+
+
+def _build(codes, precision="fp32", **kw):
+    from coherit_amd.geno_stream import GenoBlockStreamer
+    from coherit_amd.sources import ArraySource
+
+    return GenoBlockStreamer(ArraySource(int8_block(codes), missing_val=-9), precision=precision,
+                             **kw)
+
+
+@pytest.fixture(scope="module")
+def partitioned():
+    codes = random_codes(1037, 2311, seed=21)
+    sizes = [700, 1011, 600]
+    st = _build(codes, call_width=256, component_block_sizes=sizes)
+    o = geno.OraclePanel.from_codes(codes, sizes)
+    yield st, o
+    st.close()
+
+
+@pytest.mark.parametrize("k", [1, 3, 12, 20])
+@pytest.mark.parametrize("precision", ["fp32"])
+def test_kv_xtv_match_oracle(partitioned, k, precision):
+    st, o = partitioned
+    st._prec = __import__("coherit_amd._native", fromlist=["x"]).PRECISIONS[precision]
+    rng = np.random.default_rng(k)
+    V = rng.standard_normal((o.n, k)).astype(np.float32)
+    Vd = torch.as_tensor(V, device="cuda")
+    print(f"kv precision={precision} k={k}", flush=True)
+    tol = TOL[precision]
+    assert rel_err(st.kv(Vd).cpu().numpy(), okv.kv(o, V)) < tol
+    assert rel_err(st.kv(Vd, normalize=False).cpu().numpy(), okv.kv(o, V, normalize=False)) < tol
+    assert rel_err(st.xtv(Vd).cpu().numpy(), okv.xtv(o, V)) < tol
+    assert rel_err(st.xtv(Vd, normalize=True).cpu().numpy(), okv.xtv(o, V, normalize=True)) < tol
+    for g in range(3):
+        assert rel_err(st.component_kv(Vd, g).cpu().numpy(), okv.component_kv(o, V, g)) < tol
+    stacked = st.stacked_component_kv(Vd).cpu().numpy()
+    assert stacked.shape == (3, o.n, k)
+    for g in range(3):
+        assert rel_err(stacked[g], okv.component_kv(o, V, g)) < tol
+    th = np.array([0.3, 0.0, 1.7])
+    assert rel_err(st.weighted_component_hv(th, 0.45, Vd).cpu().numpy(),
+                   okv.weighted_hv(o, th, 0.45, V)) < tol
+    assert rel_err(st.weighted_component_hv(th, None, Vd).cpu().numpy(),
+                   okv.weighted_hv(o, th, None, V)) < tol
+    st._prec = 0
+
+
+def test_vector_inputs_squeeze(partitioned):
+    st, o = partitioned
+    v = np.random.default_rng(0).standard_normal(o.n).astype(np.float32)
+    out = st.kv(torch.as_tensor(v, device="cuda"))
+    assert out.shape == (o.n,)
+    assert rel_err(out.cpu().numpy(), okv.kv(o, v[:, None])[:, 0]) < 1e-5
+    xt = st.xtv(torch.as_tensor(v, device="cuda"))
+    assert xt.shape == (o.m,)
+
+
+def test_core_atoms_and_exact_diag(partitioned):
+    st, o = partitioned
+    U = np.random.default_rng(1).standard_normal((o.n, 24)).astype(np.float32)
+    Ud = torch.as_tensor(U, device="cuda")
+    got = st.build_projected_core_atoms(Ud).cpu().numpy()
+    want = okv.core_atoms(o, U)
+    for g in range(3):
+        assert rel_err(got[g] + np.eye(24), want[g] + np.eye(24)) < 1e-5
+    got1 = st.build_projected_core_atom(Ud, subtract_identity=False).cpu().numpy()
+    Z = o.standardized()
+    want1 = (Z.T @ U).T @ (Z.T @ U) / o.atoms.eff
+    assert rel_err(got1, want1) < 1e-5
+    assert rel_err(st.exact_diag().cpu().numpy(), okv.exact_diag(o)) < 1e-6
+
+
+def test_zb_matches_standardized_product(partitioned):
+    st, o = partitioned
+    b = np.random.default_rng(2).standard_normal((o.m, 2)).astype(np.float32)
+    got = st.panel.zb(torch.as_tensor(b, device="cuda"), 0, 3).cpu().numpy()
+    assert rel_err(got, o.standardized() @ b) < 1e-5
+
+
+def test_arbitrary_partition_kv():
+    codes = random_codes(333, 901, seed=8)
+    rng = np.random.default_rng(8)
+    perm = rng.permutation(901)
+    groups = [perm[:250], perm[250:]]
+    st = _build(codes, call_width=128, component_variant_indices=groups)
+    cache = cache_codes(codes, groups)
+    o = geno.OraclePanel.from_codes(cache, [250, 651])
+    V = rng.standard_normal((333, 5)).astype(np.float32)
+    Vd = torch.as_tensor(V, device="cuda")
+    for g in range(2):
+        assert rel_err(st.component_kv(Vd, g).cpu().numpy(), okv.component_kv(o, V, g)) < 1e-5
+    # sum_g eff_g K_g V == Z Z^T V (partition invariant)
+    tot = sum(float(o.atoms.component_eff[g]) * st.component_kv(Vd, g).cpu().numpy().astype(np.float64)
+              for g in range(2))
+    assert rel_err(tot, okv.kv(o, V, normalize=False)) < 1e-5
+    st.close()
+
+
+def test_errors_map_to_reference_exceptions(partitioned):
+    st, o = partitioned
+    with pytest.raises(IndexError):
+        st.component_kv(torch.zeros((o.n, 1), device="cuda"), 3)
+    with pytest.raises(ValueError):
+        st.kv(torch.zeros((o.n + 1, 2), device="cuda"))
+    with pytest.raises(ValueError):
+        st.weighted_component_hv(np.ones(2), 1.0, torch.zeros((o.n, 1), device="cuda"))

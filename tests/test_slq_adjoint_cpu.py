@@ -1,0 +1,103 @@
+"""CPU check of the hand-derived adjoint of the re-orthogonalised Lanczos step.
+
+The step/adjoint functions are device-agnostic tensor code; here they run in
+FP64 on CPU tensors against central finite differences of the forward SLQ
+value (the same check the reference applies to its JAX VJP,
+tests/test_multi_slq_score_consistency.py:54-71).
+"""
+import numpy as np
+import pytest
+import torch
+
+from coherit_amd import slq
+
+
+def problem(kind):
+    # tests/test_multi_slq_score_consistency.py:19-35 (seed 904 kernels)
+    rng = np.random.default_rng(904)
+    n = 24
+    kernels = []
+    for width in (13, 21, 37):
+        raw = rng.normal(size=(n, width))
+        kernels.append(raw @ raw.T / width)
+    if kind == "scalar":
+        kernels = [np.eye(n), 2 * np.eye(n), 3 * np.eye(n)]
+    elif kind == "clustered":
+        kernels = [np.eye(n) + 1e-4 * k for k in kernels]
+    theta = np.array([.21, .16, .28, .57])
+    if kind == "zero":
+        theta[:3] = 0
+    elif kind == "ill_conditioned":
+        theta[:] = [.8, .01, .01, .002]
+    return np.stack([*kernels, np.eye(n)]), theta
+
+
+def forward_value(directions, theta, depth, probes, with_grad=False):
+    n = directions.shape[1]
+    H = torch.as_tensor(np.einsum("k,kij->ij", theta, directions))
+    D = torch.as_tensor(directions)
+    S = probes.shape[0]
+    basis = torch.zeros((S, depth + 1, n), dtype=torch.float64)
+    basis[:, 0] = torch.as_tensor(probes) / np.sqrt(n)
+    products = torch.zeros((S, depth, n), dtype=torch.float64)
+    alphas, betas = [], []
+    prev = torch.zeros(S, dtype=torch.float64)
+    for i in range(depth):
+        prod = (H @ basis[:, i].T).T
+        nq, a, b = slq._step(basis, prod, prev, i)
+        products[:, i] = prod
+        basis[:, i + 1] = nq
+        alphas.append(a)
+        betas.append(b)
+        prev = b
+    A = torch.stack(alphas).numpy()
+    B = torch.stack(betas[:-1]).numpy() if depth > 1 else np.empty((0, S))
+    value, a_bar, b_small = slq.quadrature(A, B, n / S)
+    if not with_grad:
+        return value
+
+    def pullback(left, right):
+        prods = torch.einsum("kij,js->kis", D, left)
+        return (torch.einsum("k,kis->is", torch.as_tensor(theta), prods),
+                torch.einsum("is,kis->k", right, prods).numpy())
+
+    b_bar = np.concatenate([b_small, np.zeros((1, S))])
+    grad = slq._batch_pullback(pullback, basis, products, torch.stack(betas), a_bar, b_bar)
+    return value, grad
+
+
+@pytest.mark.parametrize("kind", ["regular", "scalar", "clustered", "zero", "ill_conditioned"])
+@pytest.mark.parametrize("depth", [1, 10, 30])
+def test_adjoint_matches_finite_differences(kind, depth):
+    directions, theta = problem(kind)
+    probes = slq.rademacher_probes(73, directions.shape[1], 4).astype(np.float64)
+    value, grad = forward_value(directions, theta, depth, probes, with_grad=True)
+    delta = 1e-6 if kind == "ill_conditioned" else 2e-5
+    fd = []
+    for k in range(len(theta)):
+        off = np.eye(len(theta))[k] * delta
+        if theta[k] == 0:
+            fd.append((forward_value(directions, theta + off, depth, probes) - value) / delta)
+        else:
+            fd.append((forward_value(directions, theta + off, depth, probes)
+                       - forward_value(directions, theta - off, depth, probes)) / (2 * delta))
+    np.testing.assert_allclose(grad, fd, rtol=2e-3, atol=3e-3)
+
+
+def test_full_depth_value_matches_dense_probe_oracle():
+    # full depth: value = sum_s z^T log(H) z / S (the dense probe oracle)
+    directions, theta = problem("regular")
+    n = directions.shape[1]
+    probes = slq.rademacher_probes(73, n, 4).astype(np.float64)
+    value, grad = forward_value(directions, theta, n, probes, with_grad=True)
+    H = np.einsum("k,kij->ij", theta, directions)
+    vals, vecs = np.linalg.eigh(H)
+    logH = (vecs * np.log(vals)) @ vecs.T
+    expected = np.einsum("sn,nm,sm->", probes, logH, probes) / 4
+    np.testing.assert_allclose(value, expected, rtol=1e-6)
+    gap = vals[:, None] - vals[None, :]
+    divided = np.divide(np.log(vals[:, None]) - np.log(vals[None, :]), gap,
+                        out=1 / np.broadcast_to(vals, gap.shape).copy(), where=np.abs(gap) > 1e-12)
+    coeff = vecs.T @ probes.T
+    gm = vecs @ (divided * (coeff @ coeff.T / 4)) @ vecs.T
+    np.testing.assert_allclose(grad, np.einsum("ij,kij->k", gm, directions), rtol=1e-5, atol=1e-6)
