@@ -33,7 +33,9 @@ FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     # IEEE-correct FP32 division and square root (bit-exact statistics)
     "-fhip-fp32-correctly-rounded-divide-sqrt",
-    "-ffp-contract=fast",
+    # FMA formation everywhere except where a pragma turns it off (the
+    # bit-exact float32 statistics epilogue)
+    "-ffp-contract=fast-honor-pragmas",
 ]
 
 

@@ -1,18 +1,639 @@
-// int8-limb MFMA operator path (filled in after the FP32 path is validated).
+// int8-limb MFMA operator path (gfx950 v_mfma_i32_16x16x64_i8).
+//
+// The centred genotype contraction is split into two exact integer
+// contractions: G0 (code value, missing -> 0) and O (observed mask):
+//   phase 1:  T_j   = s_c * ( (G0^T q)_j - mean_j (O^T q)_j )
+//   phase 2:  out_i = s_c * ( (G0 q_u)_i - (O q_w)_i ),  w_j = mean_j u_j
+// where q is the right-hand block quantised per column to fixed point
+// (|q| <= 127*2^16 as three balanced int8 limbs, or 127*2^8 as two) with
+// scale s_c.  Limb products accumulate exactly in int32 on the matrix cores;
+// limbs are combined in FP64 in the epilogue, so the only rounding is the
+// quantisation of q (relative 1.2e-7 of the column max with three limbs).
+// Integer partial sums make the split reductions order-independent.
+//
+// Operand layout: an A-lane (row r = lane%16, K group g = lane/16) loads
+// one dword = 16 consecutive samples (phase 1, variant-major rows) or 16
+// consecutive slots (phase 2, sample-major rows) and decodes it with
+// (w >> 2s) & 0x03030303 followed by a byte LUT (v_perm): register s byte b
+// holds element 16g + 4b + s.  The quantised B operand is stored in the same
+// lane order, so one 16-byte load per lane feeds one MFMA.
 #include "kv_plan.hpp"
+
+#include <algorithm>
 
 namespace coh {
 
-int plan_mfma(const coh_panel*, int precision, int64_t, int32_t, int32_t, KvPlan*) {
-  return fail(COH_ERR_VALUE, "precision %d is not available in this build", precision);
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+constexpr uint32_t kLutG0 = 0x00020100u;  // code 0,1,2,3 -> 0,1,2,0
+constexpr uint32_t kLutO = 0x00010101u;   // code 0,1,2,3 -> 1,1,1,0
+constexpr double kQ3 = 127.0 * 65536.0;
+constexpr double kQ2 = 127.0 * 256.0;
+
+__device__ inline void decode16(uint32_t w, v4i& g0, v4i& ob) {
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const uint32_t c = (w >> (2 * s)) & 0x03030303u;
+    g0[s] = static_cast<int>(__builtin_amdgcn_perm(0u, kLutG0, c));
+    ob[s] = static_cast<int>(__builtin_amdgcn_perm(0u, kLutO, c));
+  }
 }
-int run_phase1_mfma(const coh_panel*, const KvPlan&, const float*, int64_t, int64_t,
-                    const CompScale&, int, uint8_t*, hipStream_t) {
-  return fail(COH_ERR_VALUE, "MFMA path not available");
+
+// Balanced int8 limbs of a fixed-point integer q (|q| <= 127*2^(8(L-1))).
+template <int L>
+__device__ inline void limbs(int q, int out[3]) {
+  const int q0 = ((q + 128) & 255) - 128;
+  int r = (q - q0) >> 8;
+  if (L == 2) {
+    out[0] = q0;
+    out[1] = r;
+    out[2] = 0;
+    return;
+  }
+  const int q1 = ((r + 128) & 255) - 128;
+  out[0] = q0;
+  out[1] = q1;
+  out[2] = (r - q1) >> 8;
 }
-int run_phase2_mfma(const coh_panel*, const KvPlan&, int64_t, int, float*, int64_t, int64_t, float,
-                    uint8_t*, hipStream_t) {
-  return fail(COH_ERR_VALUE, "MFMA path not available");
+
+__device__ inline int quant(float x, double inv_scale, double qmax) {
+  double v = rint(static_cast<double>(x) * inv_scale);
+  v = fmin(fmax(v, -qmax), qmax);
+  return static_cast<int>(v);
+}
+
+// ---------------------------------------------------------- column scales
+
+// part[blk][c] = max_i |X[i,c]| (and |mean_i X[i,c]| when mean != null)
+__global__ __launch_bounds__(256) void k_colmax_part(int64_t rows, int64_t k,
+                                                     const float* __restrict__ X, int64_t ldx,
+                                                     const float* __restrict__ mean,
+                                                     int64_t mean_off, float* __restrict__ part) {
+  const int64_t c0 = (int64_t)blockIdx.y * 256;
+  const int kk = (int)min<int64_t>(256, k - c0);
+  const int R = 256 / kk;
+  const int t = threadIdx.x, c = t % kk, r = t / kk;
+  float mx = 0.0f;
+  if (r < R) {
+    for (int64_t i = (int64_t)blockIdx.x * R + r; i < rows; i += (int64_t)gridDim.x * R) {
+      const float x = X[i * ldx + c0 + c];
+      mx = fmaxf(mx, fabsf(x));
+      if (mean) mx = fmaxf(mx, fabsf(mean[mean_off + i] * x));
+    }
+  }
+  __shared__ float sh[256];
+  sh[t] = mx;
+  __syncthreads();
+  if (t < kk) {
+    float m = 0.0f;
+    for (int rr = 0; rr < R; ++rr) m = fmaxf(m, sh[rr * kk + t]);
+    part[(int64_t)blockIdx.x * k + c0 + t] = m;
+  }
+}
+
+__global__ void k_colmax_final(const float* __restrict__ part, int nblk, int64_t k, double qmax,
+                               double* __restrict__ scale, double* __restrict__ inv_scale) {
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < k;
+       c += (int64_t)gridDim.x * blockDim.x) {
+    float m = 0.0f;
+    for (int b = 0; b < nblk; ++b) m = fmaxf(m, part[(int64_t)b * k + c]);
+    // a non-finite column propagates as NaN (its limbs are zero)
+    const bool finite = isfinite(m);
+    const double s = (m > 0.0f && finite) ? static_cast<double>(m) / qmax : 1.0;
+    scale[c] = finite ? s : __builtin_nan("");
+    inv_scale[c] = finite ? 1.0 / s : 0.0;
+  }
+}
+
+// ------------------------------------------------------------ quantisers
+
+// Phase-1 B operand: Bq[l][nb][kb][lane][16], lane = cc + 16 g, byte 4s+b
+// holds sample kb*64 + 16g + 4b + s of column nb*16 + cc.
+template <int L>
+__global__ __launch_bounds__(256) void k_quant_p1(const float* __restrict__ V, int64_t ldv,
+                                                  int64_t n, int64_t k, int64_t KB, int64_t NBt,
+                                                  const double* __restrict__ inv_scale,
+                                                  double qmax, uint8_t* __restrict__ Bq) {
+  const int64_t total = NBt * KB * 64;
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int lane = static_cast<int>(idx % 64);
+    const int64_t kb = (idx / 64) % KB;
+    const int64_t nb = idx / (64 * KB);
+    const int cc = lane % 16, g = lane / 16;
+    const int64_t c = nb * 16 + cc;
+    uint32_t w[3][4] = {};
+    const double is = (c < k) ? inv_scale[c] : 0.0;
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const int64_t i = kb * 64 + 16 * g + 4 * b + s;
+        const float x = (c < k && i < n) ? V[i * ldv + c] : 0.0f;
+        int lm[3];
+        limbs<L>(quant(x, is, qmax), lm);
+#pragma unroll
+        for (int l = 0; l < L; ++l) w[l][s] |= (static_cast<uint32_t>(lm[l]) & 0xffu) << (8 * b);
+      }
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+      uint32_t* dst = reinterpret_cast<uint32_t*>(Bq + ((((int64_t)l * NBt + nb) * KB + kb) * 64 + lane) * 16);
+      dst[0] = w[l][0];
+      dst[1] = w[l][1];
+      dst[2] = w[l][2];
+      dst[3] = w[l][3];
+    }
+  }
+}
+
+// Phase-2 B operand: Bu[l][nb][kb][2][lane][16]: half 0 = q(u), half 1 =
+// -q(mean*u); byte 4s+b holds slot kb*64 + 16g + 4b + s (slot-local rows).
+template <int L>
+__global__ __launch_bounds__(256) void k_quant_p2(const float* __restrict__ U, int64_t ldu,
+                                                  int64_t nslots, int64_t k, int64_t KB,
+                                                  int64_t NBt, const float* __restrict__ mean,
+                                                  int64_t slot0, const double* __restrict__ inv_scale,
+                                                  double qmax, uint8_t* __restrict__ Bu) {
+  const int64_t total = NBt * KB * 64;
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int lane = static_cast<int>(idx % 64);
+    const int64_t kb = (idx / 64) % KB;
+    const int64_t nb = idx / (64 * KB);
+    const int cc = lane % 16, g = lane / 16;
+    const int64_t c = nb * 16 + cc;
+    uint32_t wu[3][4] = {}, ww[3][4] = {};
+    const double is = (c < k) ? inv_scale[c] : 0.0;
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const int64_t j = kb * 64 + 16 * g + 4 * b + s;
+        const bool ok = (c < k) && (j < nslots);
+        const float u = ok ? U[j * ldu + c] : 0.0f;
+        const float mu = ok ? mean[slot0 + j] : 0.0f;
+        int lu[3], lw[3];
+        limbs<L>(quant(u, is, qmax), lu);
+        limbs<L>(-quant(mu * u, is, qmax), lw);
+#pragma unroll
+        for (int l = 0; l < L; ++l) {
+          wu[l][s] |= (static_cast<uint32_t>(lu[l]) & 0xffu) << (8 * b);
+          ww[l][s] |= (static_cast<uint32_t>(lw[l]) & 0xffu) << (8 * b);
+        }
+      }
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+      uint8_t* base = Bu + (((((int64_t)l * NBt + nb) * KB + kb) * 2) * 64) * 16;
+      uint32_t* du = reinterpret_cast<uint32_t*>(base + lane * 16);
+      uint32_t* dw = reinterpret_cast<uint32_t*>(base + (64 + lane) * 16);
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        du[s] = wu[l][s];
+        dw[s] = ww[l][s];
+      }
+    }
+  }
+}
+
+// --------------------------------------------------------------- phase 1
+
+struct P1Args {
+  const uint8_t* vmaj;
+  int64_t vbytes;
+  int64_t slot0;       // first slot of the range
+  int64_t nslots;
+  int64_t KB;          // 64-sample blocks
+  int64_t kb_per_split;
+  int64_t NBt;         // 16-column blocks in the quantised B
+  int64_t k;
+  const uint8_t* Bq;
+  const double* scale; // per column
+  const float* mean;
+  const float* inv;
+  CompScale cs;
+  int inv_pow;
+  float* U;            // (nslots, k) when nsplit == 1
+  int32_t* part;       // [split][nslots][k][2L] when nsplit > 1
+  int nsplit;
+};
+
+template <int L, int NB, int MB>
+__global__ __launch_bounds__(256) void k_p1_mfma(P1Args a) {
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int r = lane % 16, g = lane / 16;
+  const int64_t tile0 = (int64_t)blockIdx.x * (64 * MB) + wave * (16 * MB);
+  const int nbg0 = blockIdx.z * NB;  // first 16-col block of this column group
+  const int64_t kb0 = (int64_t)blockIdx.y * a.kb_per_split;
+  const int64_t kb1 = min(a.KB, kb0 + a.kb_per_split);
+  v4i accg[L][MB][NB], acco[L][MB][NB];
+#pragma unroll
+  for (int l = 0; l < L; ++l)
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+      for (int nb = 0; nb < NB; ++nb) {
+        accg[l][mb][nb] = v4i{0, 0, 0, 0};
+        acco[l][mb][nb] = v4i{0, 0, 0, 0};
+      }
+  const uint8_t* rowp[MB];
+#pragma unroll
+  for (int mb = 0; mb < MB; ++mb)
+    rowp[mb] = a.vmaj + (a.slot0 + tile0 + mb * 16 + r) * a.vbytes + 4 * g;
+  const int64_t bstride_kb = 64 * 16;
+  const int64_t bstride_nb = a.KB * bstride_kb;
+  const int64_t bstride_l = a.NBt * bstride_nb;
+  const uint8_t* bbase = a.Bq + (int64_t)nbg0 * bstride_nb + lane * 16;
+  for (int64_t kb = kb0; kb < kb1; ++kb) {
+    uint32_t w[MB];
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb) w[mb] = *reinterpret_cast<const uint32_t*>(rowp[mb] + kb * 16);
+    v4i bq[L][NB];
+#pragma unroll
+    for (int l = 0; l < L; ++l)
+#pragma unroll
+      for (int nb = 0; nb < NB; ++nb)
+        bq[l][nb] = *reinterpret_cast<const v4i*>(bbase + l * bstride_l + nb * bstride_nb +
+                                                   kb * bstride_kb);
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb) {
+      v4i ag, ao;
+      decode16(w[mb], ag, ao);
+#pragma unroll
+      for (int l = 0; l < L; ++l)
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) {
+          accg[l][mb][nb] =
+              __builtin_amdgcn_mfma_i32_16x16x64_i8(ag, bq[l][nb], accg[l][mb][nb], 0, 0, 0);
+          acco[l][mb][nb] =
+              __builtin_amdgcn_mfma_i32_16x16x64_i8(ao, bq[l][nb], acco[l][mb][nb], 0, 0, 0);
+        }
+    }
+  }
+  // C layout: lane holds column (lane % 16), rows 4*(lane/16) + v.
+#pragma unroll
+  for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) {
+      const int64_t col = (int64_t)(nbg0 + nb) * 16 + r;
+      if (col >= a.k) continue;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int64_t jl = tile0 + mb * 16 + 4 * g + v;  // slot-local row
+        if (a.nsplit == 1) {
+          double A1 = 0.0, A2 = 0.0;
+#pragma unroll
+          for (int l = 0; l < L; ++l) {
+            A1 += ldexp(static_cast<double>(accg[l][mb][nb][v]), 8 * l);
+            A2 += ldexp(static_cast<double>(acco[l][mb][nb][v]), 8 * l);
+          }
+          const int64_t slot = a.slot0 + jl;
+          const double iv = a.inv[slot];
+          const double f = (a.inv_pow == 2 ? iv * iv : iv) * a.cs.scale_of(slot);
+          const double t = a.scale[col] * (A1 - static_cast<double>(a.mean[slot]) * A2);
+          a.U[jl * a.k + col] = static_cast<float>(f * t);
+        } else {
+          int32_t* dst = a.part + (((int64_t)blockIdx.y * a.nslots + jl) * a.k + col) * (2 * L);
+#pragma unroll
+          for (int l = 0; l < L; ++l) {
+            dst[2 * l] = accg[l][mb][nb][v];
+            dst[2 * l + 1] = acco[l][mb][nb][v];
+          }
+        }
+      }
+    }
+}
+
+template <int L>
+__global__ void k_p1_reduce(P1Args a) {
+  const int64_t total = a.nslots * a.k;
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t jl = idx / a.k, col = idx % a.k;
+    int64_t s1[3] = {0, 0, 0}, s2[3] = {0, 0, 0};
+    for (int sp = 0; sp < a.nsplit; ++sp) {
+      const int32_t* src = a.part + (((int64_t)sp * a.nslots + jl) * a.k + col) * (2 * L);
+#pragma unroll
+      for (int l = 0; l < L; ++l) {
+        s1[l] += src[2 * l];
+        s2[l] += src[2 * l + 1];
+      }
+    }
+    double A1 = 0.0, A2 = 0.0;
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+      A1 += ldexp(static_cast<double>(s1[l]), 8 * l);
+      A2 += ldexp(static_cast<double>(s2[l]), 8 * l);
+    }
+    const int64_t slot = a.slot0 + jl;
+    const double iv = a.inv[slot];
+    const double f = (a.inv_pow == 2 ? iv * iv : iv) * a.cs.scale_of(slot);
+    a.U[jl * a.k + col] =
+        static_cast<float>(f * a.scale[col] * (A1 - static_cast<double>(a.mean[slot]) * A2));
+  }
+}
+
+// --------------------------------------------------------------- phase 2
+
+struct P2Args {
+  const uint8_t* smaj;
+  int64_t sbytes;
+  int64_t n;
+  int64_t slot0;       // slot of U row 0
+  int64_t KB;          // 64-slot blocks in Bu
+  int64_t NBt;
+  int64_t k;
+  const uint8_t* Bu;
+  const double* scale;
+  ChunkTable tab;
+  int32_t* part;       // [chunk][n][k][L]
+};
+
+template <int L, int NB, int MB>
+__global__ __launch_bounds__(256) void k_p2_mfma(P2Args a) {
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int r = lane % 16, g = lane / 16;
+  const int64_t i0 = (int64_t)blockIdx.x * (64 * MB) + wave * (16 * MB);
+  const int nbg0 = blockIdx.z * NB;
+  int64_t j0, j1;
+  a.tab.chunk_range(blockIdx.y, &j0, &j1);
+  const int64_t kb0 = (j0 - a.slot0) / 64, kb1 = (j1 - a.slot0) / 64;
+  v4i acc[L][MB][NB];
+#pragma unroll
+  for (int l = 0; l < L; ++l)
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+      for (int nb = 0; nb < NB; ++nb) acc[l][mb][nb] = v4i{0, 0, 0, 0};
+  const uint8_t* rowp[MB];
+  bool rowok[MB];
+#pragma unroll
+  for (int mb = 0; mb < MB; ++mb) {
+    const int64_t i = i0 + mb * 16 + r;
+    rowok[mb] = i < a.n;
+    rowp[mb] = a.smaj + (rowok[mb] ? i : 0) * a.sbytes + a.slot0 / 4 + 4 * g;
+  }
+  const int64_t bstride_kb = 2 * 64 * 16;
+  const int64_t bstride_nb = a.KB * bstride_kb;
+  const int64_t bstride_l = a.NBt * bstride_nb;
+  const uint8_t* bbase = a.Bu + (int64_t)nbg0 * bstride_nb + lane * 16;
+  for (int64_t kb = kb0; kb < kb1; ++kb) {
+    uint32_t w[MB];
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb)
+      w[mb] = rowok[mb] ? *reinterpret_cast<const uint32_t*>(rowp[mb] + kb * 16) : 0u;
+    v4i bu[L][NB], bw[L][NB];
+#pragma unroll
+    for (int l = 0; l < L; ++l)
+#pragma unroll
+      for (int nb = 0; nb < NB; ++nb) {
+        const uint8_t* p = bbase + l * bstride_l + nb * bstride_nb + kb * bstride_kb;
+        bu[l][nb] = *reinterpret_cast<const v4i*>(p);
+        bw[l][nb] = *reinterpret_cast<const v4i*>(p + 64 * 16);
+      }
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb) {
+      v4i ag, ao;
+      decode16(w[mb], ag, ao);
+#pragma unroll
+      for (int l = 0; l < L; ++l)
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) {
+          acc[l][mb][nb] =
+              __builtin_amdgcn_mfma_i32_16x16x64_i8(ag, bu[l][nb], acc[l][mb][nb], 0, 0, 0);
+          acc[l][mb][nb] =
+              __builtin_amdgcn_mfma_i32_16x16x64_i8(ao, bw[l][nb], acc[l][mb][nb], 0, 0, 0);
+        }
+    }
+  }
+#pragma unroll
+  for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) {
+      const int64_t col = (int64_t)(nbg0 + nb) * 16 + r;
+      if (col >= a.k) continue;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int64_t i = i0 + mb * 16 + 4 * g + v;
+        if (i >= a.n) continue;
+        int32_t* dst = a.part + (((int64_t)blockIdx.y * a.n + i) * a.k + col) * L;
+#pragma unroll
+        for (int l = 0; l < L; ++l) dst[l] = acc[l][mb][nb][v];
+      }
+    }
+}
+
+template <int L>
+__global__ void k_p2_reduce(const int32_t* __restrict__ part, int64_t n, int64_t k,
+                            const double* __restrict__ scale, ChunkTable tab, int nout, int stacked,
+                            float* __restrict__ out, int64_t ldo, int64_t out_stride, float beta) {
+  const int64_t total = (int64_t)nout * n * k;
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t col = idx % k;
+    const int64_t i = (idx / k) % n;
+    const int go = static_cast<int>(idx / (k * n));
+    int cb = 0, ce = tab.nchunks;
+    if (stacked) {
+      cb = tab.chunk_off[go];
+      ce = tab.chunk_off[go + 1];
+    }
+    int64_t s[3] = {0, 0, 0};
+    for (int ch = cb; ch < ce; ++ch) {
+      const int32_t* src = part + (((int64_t)ch * n + i) * k + col) * L;
+#pragma unroll
+      for (int l = 0; l < L; ++l) s[l] += src[l];
+    }
+    double t = 0.0;
+#pragma unroll
+    for (int l = 0; l < L; ++l) t += ldexp(static_cast<double>(s[l]), 8 * l);
+    const float v = static_cast<float>(scale[col] * t);
+    float* o = out + (int64_t)go * out_stride + i * ldo + col;
+    *o = (beta != 0.0f) ? beta * (*o) + v : v;
+  }
+}
+
+// ------------------------------------------------------------- planning
+
+static unsigned grid1d(int64_t total) {
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(total, 256), 8192));
+}
+
+struct MfmaShape {
+  int NB, MB1, MB2;
+};
+
+static MfmaShape shape_for(int64_t k) {
+  if (k <= 16) return {1, 4, 8};
+  if (k <= 32) return {2, 2, 4};
+  return {4, 1, 2};
+}
+
+constexpr int kColmaxBlocks = 256;
+
+int plan_mfma(const coh_panel* p, int precision, int64_t k, int32_t cb, int32_t ce, KvPlan* pl) {
+  pl->limbs = (precision == COH_PREC_I8X3) ? 3 : 2;
+  pl->slot0 = p->comp_slot_begin[cb];
+  pl->slot1 = p->comp_slot_end[ce - 1];
+  const int64_t nslots = pl->slot1 - pl->slot0;
+  const MfmaShape sh = shape_for(k);
+  pl->KC = sh.NB * 16;
+  const int64_t groups = ceil_div(ceil_div(k, 16), sh.NB);
+  const int64_t NBt = groups * sh.NB;  // column blocks padded to whole groups (zero-filled)
+  const int64_t KB1 = ceil_div(p->n, 64);
+  pl->n_pad = KB1 * 64;
+  // phase-1 sample splits only when there are too few slot tiles
+  const int64_t tiles1 = std::max<int64_t>(1, nslots / (64 * sh.MB1));
+  int64_t nsplit = std::max<int64_t>(1, ceil_div(kTargetBlocks, tiles1 * groups));
+  nsplit = std::min<int64_t>(nsplit, std::max<int64_t>(1, KB1 / 8));
+  nsplit = std::min<int64_t>(nsplit, 64);
+  pl->nsplit1 = static_cast<int>(nsplit);
+  pl->rows_per_split = ceil_div(KB1, nsplit);  // in 64-sample blocks
+  // phase-2 slot chunks
+  const int64_t xb2 = std::max<int64_t>(1, ceil_div(p->n, 64 * sh.MB2)) * groups;
+  const int64_t target_chunks = std::max<int64_t>(1, ceil_div(kTargetBlocks, xb2));
+  pl->chunk_len = std::max<int64_t>(256, round_up(ceil_div(nslots, target_chunks), 256));
+  build_chunks(p, cb, ce, pl->chunk_len, &pl->tab);
+  pl->ldu = k;
+  const int L = pl->limbs;
+  int64_t off = 0;
+  pl->off_U = off;
+  off += round_up(nslots * k * 4, 256);
+  pl->off_q = off;  // Bq (phase 1) then Bu (phase 2) share nothing: both kept
+  off += round_up((int64_t)L * NBt * KB1 * 64 * 16, 256);
+  pl->off_p2 = off;  // Bu
+  off += round_up((int64_t)L * NBt * (nslots / 64) * 2 * 64 * 16, 256);
+  pl->off_aux = off;  // scales: 2 x k doubles, colmax partials
+  off += round_up(2 * k * 8, 256) + round_up((int64_t)kColmaxBlocks * k * 4, 256);
+  pl->off_p1 = off;
+  int64_t p1 = (nsplit > 1) ? nsplit * nslots * k * 2 * L * 4 : 0;
+  int64_t p2 = (int64_t)pl->tab.nchunks * p->n * k * L * 4;
+  off += round_up(std::max(p1, p2), 256);
+  pl->total = off;
+  return COH_OK;
+}
+
+static int colscale(const float* X, int64_t rows, int64_t k, int64_t ldx, const float* mean,
+                    int64_t mean_off, double qmax, uint8_t* aux, hipStream_t st) {
+  double* scale = reinterpret_cast<double*>(aux);
+  double* inv_scale = scale + k;
+  float* part = reinterpret_cast<float*>(aux + round_up(2 * k * 8, 256));
+  const int nblk = (int)std::max<int64_t>(1, std::min<int64_t>(kColmaxBlocks, ceil_div(rows * std::min<int64_t>(k, 256), 4096)));
+  hipLaunchKernelGGL(k_colmax_part, dim3(nblk, (unsigned)ceil_div(k, 256)), dim3(256), 0, st, rows,
+                     k, X, ldx, mean, mean_off, part);
+  COH_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_colmax_final, dim3((unsigned)ceil_div(k, 256)), dim3(256), 0, st, part, nblk,
+                     k, qmax, scale, inv_scale);
+  COH_CHECK_LAUNCH();
+  return COH_OK;
+}
+
+template <int L>
+static int phase1(const coh_panel* p, const KvPlan& pl, const float* V, int64_t k, int64_t ldv,
+                  const CompScale& cs, int inv_pow, uint8_t* ws, hipStream_t st) {
+  const double qmax = (L == 3) ? kQ3 : kQ2;
+  const int64_t nslots = pl.slot1 - pl.slot0;
+  const MfmaShape sh = shape_for(k);
+  const int64_t groups = ceil_div(ceil_div(k, 16), sh.NB);
+  const int64_t NBt = groups * sh.NB;
+  const int64_t KB = pl.n_pad / 64;
+  uint8_t* aux = ws + pl.off_aux;
+  int s = colscale(V, p->n, k, ldv, nullptr, 0, qmax, aux, st);
+  if (s != COH_OK) return s;
+  const double* scale = reinterpret_cast<const double*>(aux);
+  const double* inv_scale = scale + k;
+  uint8_t* Bq = ws + pl.off_q;
+  hipLaunchKernelGGL(k_quant_p1<L>, dim3(grid1d(NBt * KB * 64)), dim3(256), 0, st, V, ldv, p->n,
+                     k, KB, NBt, inv_scale, qmax, Bq);
+  COH_CHECK_LAUNCH();
+  P1Args a;
+  a.vmaj = p->vmaj;
+  a.vbytes = p->vbytes;
+  a.slot0 = pl.slot0;
+  a.nslots = nslots;
+  a.KB = KB;
+  a.kb_per_split = pl.rows_per_split;
+  a.NBt = NBt;
+  a.k = k;
+  a.Bq = Bq;
+  a.scale = scale;
+  a.mean = p->mean;
+  a.inv = p->inv;
+  a.cs = cs;
+  a.inv_pow = inv_pow;
+  a.U = reinterpret_cast<float*>(ws + pl.off_U);
+  a.part = reinterpret_cast<int32_t*>(ws + pl.off_p1);
+  a.nsplit = pl.nsplit1;
+  dim3 grid((unsigned)(nslots / (64 * sh.MB1)), (unsigned)pl.nsplit1, (unsigned)groups);
+  if (sh.NB == 1)
+    hipLaunchKernelGGL((k_p1_mfma<L, 1, 4>), grid, dim3(256), 0, st, a);
+  else if (sh.NB == 2)
+    hipLaunchKernelGGL((k_p1_mfma<L, 2, 2>), grid, dim3(256), 0, st, a);
+  else
+    hipLaunchKernelGGL((k_p1_mfma<L, 4, 1>), grid, dim3(256), 0, st, a);
+  COH_CHECK_LAUNCH();
+  if (pl.nsplit1 > 1) {
+    hipLaunchKernelGGL(k_p1_reduce<L>, dim3(grid1d(nslots * k)), dim3(256), 0, st, a);
+    COH_CHECK_LAUNCH();
+  }
+  return COH_OK;
+}
+
+template <int L>
+static int phase2(const coh_panel* p, const KvPlan& pl, int64_t k, int stacked, float* out,
+                  int64_t ldo, int64_t out_stride, float beta, uint8_t* ws, hipStream_t st) {
+  const double qmax = (L == 3) ? kQ3 : kQ2;
+  const int64_t nslots = pl.slot1 - pl.slot0;
+  const MfmaShape sh = shape_for(k);
+  const int64_t groups = ceil_div(ceil_div(k, 16), sh.NB);
+  const int64_t NBt = groups * sh.NB;
+  const int64_t KB = nslots / 64;
+  const float* U = reinterpret_cast<const float*>(ws + pl.off_U);
+  uint8_t* aux = ws + pl.off_aux;
+  int s = colscale(U, nslots, k, pl.ldu, p->mean, pl.slot0, qmax, aux, st);
+  if (s != COH_OK) return s;
+  const double* scale = reinterpret_cast<const double*>(aux);
+  const double* inv_scale = scale + k;
+  uint8_t* Bu = ws + pl.off_p2;
+  hipLaunchKernelGGL(k_quant_p2<L>, dim3(grid1d(NBt * KB * 64)), dim3(256), 0, st, U, pl.ldu,
+                     nslots, k, KB, NBt, p->mean, pl.slot0, inv_scale, qmax, Bu);
+  COH_CHECK_LAUNCH();
+  P2Args a;
+  a.smaj = p->smaj;
+  a.sbytes = p->sbytes;
+  a.n = p->n;
+  a.slot0 = pl.slot0;
+  a.KB = KB;
+  a.NBt = NBt;
+  a.k = k;
+  a.Bu = Bu;
+  a.scale = scale;
+  a.tab = pl.tab;
+  a.part = reinterpret_cast<int32_t*>(ws + pl.off_p1);
+  dim3 grid((unsigned)ceil_div(p->n, 64 * sh.MB2), (unsigned)pl.tab.nchunks, (unsigned)groups);
+  if (sh.NB == 1)
+    hipLaunchKernelGGL((k_p2_mfma<L, 1, 8>), grid, dim3(256), 0, st, a);
+  else if (sh.NB == 2)
+    hipLaunchKernelGGL((k_p2_mfma<L, 2, 4>), grid, dim3(256), 0, st, a);
+  else
+    hipLaunchKernelGGL((k_p2_mfma<L, 4, 2>), grid, dim3(256), 0, st, a);
+  COH_CHECK_LAUNCH();
+  const int nout = stacked ? pl.tab.ncomp : 1;
+  hipLaunchKernelGGL(k_p2_reduce<L>, dim3(grid1d((int64_t)nout * p->n * k)), dim3(256), 0, st,
+                     a.part, p->n, k, scale, pl.tab, nout, stacked, out, ldo, out_stride, beta);
+  COH_CHECK_LAUNCH();
+  return COH_OK;
+}
+
+int run_phase1_mfma(const coh_panel* p, const KvPlan& pl, const float* V, int64_t k, int64_t ldv,
+                    const CompScale& cs, int inv_pow, uint8_t* ws, hipStream_t st) {
+  return pl.limbs == 3 ? phase1<3>(p, pl, V, k, ldv, cs, inv_pow, ws, st)
+                       : phase1<2>(p, pl, V, k, ldv, cs, inv_pow, ws, st);
+}
+
+int run_phase2_mfma(const coh_panel* p, const KvPlan& pl, int64_t k, int stacked, float* out,
+                    int64_t ldo, int64_t out_stride, float beta, uint8_t* ws, hipStream_t st) {
+  return pl.limbs == 3 ? phase2<3>(p, pl, k, stacked, out, ldo, out_stride, beta, ws, st)
+                       : phase2<2>(p, pl, k, stacked, out, ldo, out_stride, beta, ws, st);
 }
 
 }  // namespace coh

@@ -112,7 +112,7 @@ __global__ __launch_bounds__(256) void k_ingest_bed(const uint8_t* __restrict__ 
       if (i >= n) break;
       const int64_t fi = sample_idx[i];
       const uint32_t g = (row[fi >> 2] >> ((fi & 3) << 1)) & 3u;
-      const uint32_t code = (0x1Bu >> (2 * g)) & 3u;  // [2,3,1,0] packed: 2 | 3<<2 | 1<<4 | 0<<6
+      const uint32_t code = (0x1Eu >> (2 * g)) & 3u;  // [2,3,1,0] packed: 2 | 3<<2 | 1<<4 | 0<<6
       packed |= code << (2 * t);
     }
     vmaj[(int64_t)cache_to_slot[cache_start + r] * vbytes + b] = static_cast<uint8_t>(packed);
@@ -246,7 +246,13 @@ __device__ void f32_stats(int64_t c, int64_t a1, int64_t a2, float* mean_out,
   const float var = fmaxf(q - sq, 0.0f);
   const bool valid = (cnt_f > 0.0f) && (var > 0.0f);
   float inv = 0.0f;
-  if (valid) inv = 1.0f / sqrtf(fmaxf(var, 1e-6f));
+  if (valid) {
+    // Correctly rounded FP32 sqrt and reciprocal: evaluate in FP64 and round
+    // each result to FP32 (double rounding from 53 to 24 bits is innocuous
+    // for sqrt and division since 53 >= 2*24 + 2), matching NumPy float32.
+    const float t = static_cast<float>(__builtin_sqrt(static_cast<double>(fmaxf(var, 1e-6f))));
+    inv = static_cast<float>(1.0 / static_cast<double>(t));
+  }
   *mean_out = mean;
   *inv_out = inv;
 }

@@ -398,6 +398,13 @@ class GenoBlockStreamer:
         b = int(self._component_snp_offsets[component_idx + 1])
         return np.asarray(self._cache_to_source_variant_indices[a:b], dtype=np.int64)
 
+    def set_precision(self, precision: str) -> None:
+        """Switch the contraction kernels ("fp32", "i8x3", "i8x2")."""
+        if precision not in N.PRECISIONS:
+            raise ValueError(f"precision must be one of {sorted(N.PRECISIONS)}, got {precision!r}")
+        self.precision = precision
+        self._prec = N.PRECISIONS[precision]
+
     def _check_component(self, component_idx):
         if component_idx < 0 or component_idx >= self._n_components:
             raise IndexError(

@@ -9,7 +9,8 @@ from tests.helpers import cache_codes, int8_block, random_codes, rel_err
 
 pytestmark = pytest.mark.gpu
 
-TOL = {"fp32": 1e-5}
+# plain FP32 and exact 24-bit limbs: 1e-5 relative; 16-bit limbs: 2e-4 relative
+TOL = {"fp32": 1e-5, "i8x3": 1e-5, "i8x2": 2e-4}
 
 
 def _build(codes, precision="fp32", **kw):
@@ -31,10 +32,10 @@ def partitioned():
 
 
 @pytest.mark.parametrize("k", [1, 3, 12, 20])
-@pytest.mark.parametrize("precision", ["fp32"])
+@pytest.mark.parametrize("precision", ["fp32", "i8x3", "i8x2"])
 def test_kv_xtv_match_oracle(partitioned, k, precision):
     st, o = partitioned
-    st._prec = __import__("coherit_amd._native", fromlist=["x"]).PRECISIONS[precision]
+    st.set_precision(precision)
     rng = np.random.default_rng(k)
     V = rng.standard_normal((o.n, k)).astype(np.float32)
     Vd = torch.as_tensor(V, device="cuda")
@@ -55,7 +56,7 @@ def test_kv_xtv_match_oracle(partitioned, k, precision):
                    okv.weighted_hv(o, th, 0.45, V)) < tol
     assert rel_err(st.weighted_component_hv(th, None, Vd).cpu().numpy(),
                    okv.weighted_hv(o, th, None, V)) < tol
-    st._prec = 0
+    st.set_precision("fp32")
 
 
 def test_vector_inputs_squeeze(partitioned):
@@ -68,8 +69,10 @@ def test_vector_inputs_squeeze(partitioned):
     assert xt.shape == (o.m,)
 
 
-def test_core_atoms_and_exact_diag(partitioned):
+@pytest.mark.parametrize("precision", ["fp32", "i8x3"])
+def test_core_atoms_and_exact_diag(partitioned, precision):
     st, o = partitioned
+    st.set_precision(precision)
     U = np.random.default_rng(1).standard_normal((o.n, 24)).astype(np.float32)
     Ud = torch.as_tensor(U, device="cuda")
     got = st.build_projected_core_atoms(Ud).cpu().numpy()
@@ -81,13 +84,29 @@ def test_core_atoms_and_exact_diag(partitioned):
     want1 = (Z.T @ U).T @ (Z.T @ U) / o.atoms.eff
     assert rel_err(got1, want1) < 1e-5
     assert rel_err(st.exact_diag().cpu().numpy(), okv.exact_diag(o)) < 1e-6
+    st.set_precision("fp32")
 
 
-def test_zb_matches_standardized_product(partitioned):
+@pytest.mark.parametrize("precision", [0, 1])
+def test_zb_matches_standardized_product(partitioned, precision):
     st, o = partitioned
     b = np.random.default_rng(2).standard_normal((o.m, 2)).astype(np.float32)
-    got = st.panel.zb(torch.as_tensor(b, device="cuda"), 0, 3).cpu().numpy()
+    got = st.panel.zb(torch.as_tensor(b, device="cuda"), 0, 3, precision=precision).cpu().numpy()
     assert rel_err(got, o.standardized() @ b) < 1e-5
+
+
+@pytest.mark.parametrize("k", [1, 16, 17, 64, 65])
+def test_mfma_exact_limbs_cover_column_groups_and_sample_splits(k):
+    # small n forces phase-1 sample splits; k spans 1..5 column blocks
+    codes = random_codes(131, 700, seed=k)
+    st = _build(codes, precision="i8x3", call_width=300, component_block_sizes=[300, 400])
+    o = geno.OraclePanel.from_codes(codes, [300, 400])
+    V = np.random.default_rng(k).standard_normal((131, k)).astype(np.float32)
+    Vd = torch.as_tensor(V, device="cuda")
+    print(f"mfma k={k}", flush=True)
+    assert rel_err(st.stacked_component_kv(Vd).cpu().numpy(), okv.stacked_kv(o, V)) < 1e-5
+    assert rel_err(st.xtv(Vd).cpu().numpy(), okv.xtv(o, V)) < 1e-5
+    st.close()
 
 
 def test_arbitrary_partition_kv():

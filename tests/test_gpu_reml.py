@@ -1,0 +1,123 @@
+"""GPU: full REML fits match the FP64 oracle fit with identical probes.
+
+Tolerances: |dh2| <= 1e-5 and theta within 1e-4 relative for the plain
+FP32 path; SLQ log-determinants within 1e-5 relative.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import geno
+from oracle import reml as oreml
+from tests.helpers import int8_block, random_codes
+
+pytestmark = pytest.mark.gpu
+
+
+def _panel(n, m, sizes, seed, precision="fp32", call_width=512):
+    from coherit_amd.geno_stream import GenoBlockStreamer
+    from coherit_amd.sources import ArraySource
+
+    codes = random_codes(n, m, seed=seed)
+    st = GenoBlockStreamer(ArraySource(int8_block(codes), missing_val=-9), call_width=call_width,
+                           component_block_sizes=sizes, precision=precision)
+    o = geno.OraclePanel.from_codes(codes, sizes)
+    return st, o
+
+
+def _dense(o):
+    Z = o.standardized()
+    off = o.component_offsets
+    Ks = []
+    for g in range(len(o.component_sizes)):
+        Zg = Z[:, off[g]:off[g + 1]]
+        Ks.append(Zg @ Zg.T / float(o.atoms.component_eff[g]))
+    return Ks
+
+
+def _pheno(o, seed, h2=(0.3, 0.2)):
+    rng = np.random.default_rng(seed)
+    n = o.n
+    X = np.concatenate([np.ones((n, 1)), rng.standard_normal((n, 2))], axis=1)
+    Z = o.standardized()
+    off = o.component_offsets
+    y = X @ np.array([0.5, 1.0, -0.5])
+    for g in range(len(o.component_sizes)):
+        b = rng.standard_normal(off[g + 1] - off[g])
+        zb = Z[:, off[g]:off[g + 1]] @ b
+        y = y + np.sqrt(h2[g]) * zb / zb.std()
+    y = y + np.sqrt(1 - sum(h2[: len(o.component_sizes)])) * rng.standard_normal(n)
+    return y.astype(np.float32), X.astype(np.float32)
+
+
+def _h2(theta, atoms):
+    G = len(atoms)
+    gen = float(np.dot(theta[:G], atoms))
+    return gen / (gen + float(theta[G]))
+
+
+def test_affine_single_grm_fit_matches_oracle():
+    from coherit_amd.reml import fit_reml
+
+    st, o = _panel(360, 1400, None, seed=31)
+    y, X = _pheno(o, 5, h2=(0.5,))
+    kw = dict(seed=3, slq_samples=12, slq_m=16, minq_iter=20, pcg_tol=1e-6)
+    theta, hist = fit_reml(y, [lambda V: st.kv(V)], [st.diag()], X, 10, 400, verbose=False, **kw)
+    pb = oreml.Problem(_dense(o), o.atoms.component_mean_diag.astype(np.float64))
+    th_o = oreml.fit_reml(pb, y, X, maxiter=400, **kw)
+    print("theta", theta, "oracle", th_o, flush=True)
+    atoms = o.atoms.component_mean_diag.astype(np.float64)
+    assert abs(_h2(theta, atoms) - _h2(th_o, atoms)) <= 1e-5
+    np.testing.assert_allclose(theta, th_o, rtol=1e-4)
+    st.close()
+
+
+@pytest.mark.parametrize("precision", ["fp32", "i8x3"])
+def test_two_component_multi_slq_fit_matches_oracle(precision):
+    from coherit_amd.reml import fit_reml
+
+    st, o = _panel(300, 1500, [600, 900], seed=32, precision=precision)
+    y, X = _pheno(o, 6)
+    # pcg_tol 1e-5: the tolerance the reference forces for multi-GRM fits (reml.py:1400-1408)
+    kw = dict(seed=5, slq_samples=8, slq_m=12, minq_iter=12, pcg_tol=1e-5)
+    K_mvs = [lambda V, g=g: st.component_kv(V, g) for g in range(2)]
+    counters = {}
+    theta, hist = fit_reml(y, K_mvs, st.component_diag_list(), X, 10, 400, verbose=False,
+                           weighted_hv=st.weighted_component_hv, stacked_kv=st.stacked_component_kv,
+                           counters=counters, **kw)
+    pb = oreml.Problem(_dense(o), o.atoms.component_mean_diag.astype(np.float64))
+    th_o, hist_o, info = oreml.fit_reml(pb, y, X, maxiter=400, return_history=True, **kw)
+    print("theta", theta, "oracle", th_o, counters, flush=True)
+    for h in hist:
+        print("gpu   ", h["iter"], h["status"], f"{h['loglik']:.10f}", h["params"],
+              [(t["alpha"], t["dll"]) for t in h["line_search_trace"]], flush=True)
+    for h in hist_o:
+        print("oracle", h["iter"], h["status"], h.get("loglik"), h["params"], flush=True)
+    atoms = o.atoms.component_mean_diag.astype(np.float64)
+    assert abs(_h2(theta, atoms) - _h2(th_o, atoms)) <= 1e-5
+    np.testing.assert_allclose(theta, th_o, rtol=1e-4, atol=1e-7)
+    st.close()
+
+
+def test_fitter_with_projected_core_whitening_matches_oracle():
+    from coherit_amd.reml_model import FitConfig, InfinitesimalREMLFitter
+    from coherit_amd.sources import ArraySource
+
+    codes = random_codes(320, 1200, seed=33)
+    sizes = [500, 700]
+    cfg = FitConfig(sources=[ArraySource(int8_block(codes), missing_val=-9)], vc_block_sizes=sizes,
+                    call_width=400, precond_rank=40, slq_samples=8, slq_m=12, minq_iter=12,
+                    reml_pcg_tol=1e-5, seed=7, verbose=False)
+    fitter = InfinitesimalREMLFitter(cfg)
+    o = geno.OraclePanel.from_codes(codes, sizes)
+    y, X = _pheno(o, 7)
+    res = fitter.fit_infinitesimal(y, X)
+    U = fitter.slq_precond_conf.U.double().cpu().numpy()
+    pb = oreml.Problem(_dense(o), o.atoms.component_mean_diag.astype(np.float64))
+    th_o = oreml.fit_reml(pb, y, X, seed=7, slq_samples=8, slq_m=12, minq_iter=12, pcg_tol=1e-5,
+                          slq_mode="projected_core_residual", precond_U=U)
+    print("theta", res.var_components, "oracle", th_o, "h2", res.heritability, flush=True)
+    atoms = o.atoms.component_mean_diag.astype(np.float64)
+    assert abs(res.heritability - _h2(th_o, atoms)) <= 1e-5
+    np.testing.assert_allclose(res.var_components, th_o, rtol=1e-4, atol=1e-7)
+    fitter.close()
