@@ -72,6 +72,9 @@ SIGNATURES = {
     "coh_pcg_direction_update": [_i64, _i64, _p, _i64, _p, _i64, _p, _i64, _p, _p, _p, _p, _p,
                                  _u64, _p],
     "coh_lanczos_step": [_i64, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _u64, _p],
+    "coh_timing_enable": [_i32],
+    "coh_timing_reset": [],
+    "coh_timing_read": [_i32, _p, _p, _p, _p, _p, _p, _p],
 }
 _RESTYPE = {"coh_last_error": ctypes.c_char_p}
 
@@ -136,3 +139,29 @@ def stream_ptr(stream=None) -> int:
 
     s = torch.cuda.current_stream() if stream is None else stream
     return s.cuda_stream
+
+
+def timing_enable(on: bool = True) -> None:
+    """Turn the library's per-kernel HIP-event instrumentation on or off."""
+    call("coh_timing_enable", 1 if on else 0)
+
+
+def timing_reset() -> None:
+    call("coh_timing_reset")
+
+
+def timing_read(max_kernels: int = 64) -> dict:
+    """{kernel: {launches, ms, bytes, flops, ops}} accumulated since reset."""
+    import numpy as np
+
+    n = ctypes.c_int32()
+    names = ctypes.create_string_buffer(64 * max_kernels)
+    launches = np.zeros(max_kernels, np.int64)
+    arrs = [np.zeros(max_kernels, np.float64) for _ in range(4)]
+    call("coh_timing_read", max_kernels, ctypes.byref(n), names, ptr(launches), *(ptr(a) for a in arrs))
+    out = {}
+    for i in range(min(n.value, max_kernels)):
+        name = names.raw[64 * i : 64 * (i + 1)].split(b"\0", 1)[0].decode()
+        out[name] = dict(launches=int(launches[i]), ms=float(arrs[0][i]), bytes=float(arrs[1][i]),
+                         flops=float(arrs[2][i]), ops=float(arrs[3][i]))
+    return out

@@ -10,16 +10,36 @@
 // blocks and reduce the partials in a fixed order (deterministic results).
 //
 // Precision COH_PREC_FP32 runs on the VALU: the centred value is formed in
-// FP32 exactly as the reference does and accumulated with FP32 FMAs.  The
-// int8-limb MFMA precisions live in kv_mfma.hip.
+// FP32 exactly as the reference does and multiplied in FP32; FMAs accumulate
+// in FP32 over runs of kRun terms that are flushed into FP64 accumulators, and
+// split partials are reduced in FP64.  A plain FP32 running sum over ~1e5
+// samples loses ~1e-2 absolute on sums of centred codes whose exact value is
+// ~1e-3 (the intercept column of X^T V), which puts a ~1e-4 floor under PCG
+// residuals at n = 1e5; the blocked sum removes it at ~3% extra VALU work.
+// The int8-limb MFMA precisions live in kv_mfma.hip.
 #include "common.hpp"
 #include "kv_plan.hpp"
+#include "timing.hpp"
 
 #include <algorithm>
+#include <cstdio>
 
 namespace coh {
 
 // ------------------------------------------------------------ VALU kernels
+
+constexpr int kRun = 32;  // FP32 accumulation run before an FP64 flush
+
+template <int KC>
+__device__ inline void flush_run(double tot[4][KC], float acc[4][KC]) {
+#pragma unroll
+  for (int s = 0; s < 4; ++s)
+#pragma unroll
+    for (int c = 0; c < KC; ++c) {
+      tot[s][c] += static_cast<double>(acc[s][c]);
+      acc[s][c] = 0.0f;
+    }
+}
 
 template <int KC>
 __device__ inline void p1_accum(uint32_t b, const float* __restrict__ vrow, int kvalid,
@@ -43,7 +63,7 @@ __global__ __launch_bounds__(256) void k_p1_valu(const uint8_t* __restrict__ sma
                                                  const float* __restrict__ mean,
                                                  const float* __restrict__ V, int64_t ldv,
                                                  int kvalid, int64_t rows_per_split,
-                                                 float* __restrict__ part) {
+                                                 double* __restrict__ part) {
   const int64_t qb = (int64_t)blockIdx.x * 256 + threadIdx.x;
   const bool act = qb < nbytes;
   const int64_t q = byte0 + (act ? qb : 0);
@@ -53,29 +73,37 @@ __global__ __launch_bounds__(256) void k_p1_valu(const uint8_t* __restrict__ sma
 #pragma unroll
   for (int s = 0; s < 4; ++s) mu[s] = mean[4 * q + s];
   float acc[4][KC];
+  double tot[4][KC];
 #pragma unroll
   for (int s = 0; s < 4; ++s)
 #pragma unroll
-    for (int c = 0; c < KC; ++c) acc[s][c] = 0.0f;
+    for (int c = 0; c < KC; ++c) {
+      acc[s][c] = 0.0f;
+      tot[s][c] = 0.0;
+    }
   const uint8_t* col = smaj + q;
   int64_t i = i0;
-  for (; i + 4 <= i1; i += 4) {
-    const uint32_t b0 = col[(i + 0) * sbytes];
-    const uint32_t b1 = col[(i + 1) * sbytes];
-    const uint32_t b2 = col[(i + 2) * sbytes];
-    const uint32_t b3 = col[(i + 3) * sbytes];
-    p1_accum<KC>(b0, V + (i + 0) * ldv, kvalid, mu, acc);
-    p1_accum<KC>(b1, V + (i + 1) * ldv, kvalid, mu, acc);
-    p1_accum<KC>(b2, V + (i + 2) * ldv, kvalid, mu, acc);
-    p1_accum<KC>(b3, V + (i + 3) * ldv, kvalid, mu, acc);
+  while (i < i1) {
+    const int64_t iend = min(i1, i + kRun);
+    for (; i + 4 <= iend; i += 4) {
+      const uint32_t b0 = col[(i + 0) * sbytes];
+      const uint32_t b1 = col[(i + 1) * sbytes];
+      const uint32_t b2 = col[(i + 2) * sbytes];
+      const uint32_t b3 = col[(i + 3) * sbytes];
+      p1_accum<KC>(b0, V + (i + 0) * ldv, kvalid, mu, acc);
+      p1_accum<KC>(b1, V + (i + 1) * ldv, kvalid, mu, acc);
+      p1_accum<KC>(b2, V + (i + 2) * ldv, kvalid, mu, acc);
+      p1_accum<KC>(b3, V + (i + 3) * ldv, kvalid, mu, acc);
+    }
+    for (; i < iend; ++i) p1_accum<KC>(col[i * sbytes], V + i * ldv, kvalid, mu, acc);
+    flush_run<KC>(tot, acc);
   }
-  for (; i < i1; ++i) p1_accum<KC>(col[i * sbytes], V + i * ldv, kvalid, mu, acc);
   if (!act) return;
-  float* out = part + ((int64_t)blockIdx.y * nbytes + qb) * 4 * KC;
+  double* out = part + ((int64_t)blockIdx.y * nbytes + qb) * 4 * KC;
 #pragma unroll
   for (int s = 0; s < 4; ++s)
 #pragma unroll
-    for (int c = 0; c < KC; ++c) out[s * KC + c] = acc[s][c];
+    for (int c = 0; c < KC; ++c) out[s * KC + c] = tot[s][c];
 }
 
 template <int KC>
@@ -99,40 +127,49 @@ __global__ __launch_bounds__(256) void k_p2_valu(const uint8_t* __restrict__ vma
                                                  int64_t n, const float* __restrict__ mean,
                                                  const float* __restrict__ U, int64_t ldu,
                                                  int64_t slot_base, int kvalid, ChunkTable tab,
-                                                 float* __restrict__ part) {
+                                                 double* __restrict__ part) {
   const int64_t nb = (n + 3) / 4;
   const int64_t b = (int64_t)blockIdx.x * 256 + threadIdx.x;
   const bool act = b < nb;
   int64_t j0, j1;
   tab.chunk_range(blockIdx.y, &j0, &j1);
   float acc[4][KC];
+  double tot[4][KC];
 #pragma unroll
   for (int r = 0; r < 4; ++r)
 #pragma unroll
-    for (int c = 0; c < KC; ++c) acc[r][c] = 0.0f;
+    for (int c = 0; c < KC; ++c) {
+      acc[r][c] = 0.0f;
+      tot[r][c] = 0.0;
+    }
   const uint8_t* col = vmaj + (act ? b : 0);
   int64_t j = j0;
-  for (; j + 4 <= j1; j += 4) {
-    const uint32_t y0 = col[(j + 0) * vbytes];
-    const uint32_t y1 = col[(j + 1) * vbytes];
-    const uint32_t y2 = col[(j + 2) * vbytes];
-    const uint32_t y3 = col[(j + 3) * vbytes];
-    p2_accum<KC>(y0, mean[j + 0], U + (j + 0 - slot_base) * ldu, kvalid, acc);
-    p2_accum<KC>(y1, mean[j + 1], U + (j + 1 - slot_base) * ldu, kvalid, acc);
-    p2_accum<KC>(y2, mean[j + 2], U + (j + 2 - slot_base) * ldu, kvalid, acc);
-    p2_accum<KC>(y3, mean[j + 3], U + (j + 3 - slot_base) * ldu, kvalid, acc);
+  while (j < j1) {
+    const int64_t jend = min(j1, j + kRun);
+    for (; j + 4 <= jend; j += 4) {
+      const uint32_t y0 = col[(j + 0) * vbytes];
+      const uint32_t y1 = col[(j + 1) * vbytes];
+      const uint32_t y2 = col[(j + 2) * vbytes];
+      const uint32_t y3 = col[(j + 3) * vbytes];
+      p2_accum<KC>(y0, mean[j + 0], U + (j + 0 - slot_base) * ldu, kvalid, acc);
+      p2_accum<KC>(y1, mean[j + 1], U + (j + 1 - slot_base) * ldu, kvalid, acc);
+      p2_accum<KC>(y2, mean[j + 2], U + (j + 2 - slot_base) * ldu, kvalid, acc);
+      p2_accum<KC>(y3, mean[j + 3], U + (j + 3 - slot_base) * ldu, kvalid, acc);
+    }
+    for (; j < jend; ++j)
+      p2_accum<KC>(col[j * vbytes], mean[j], U + (j - slot_base) * ldu, kvalid, acc);
+    flush_run<KC>(tot, acc);
   }
-  for (; j < j1; ++j) p2_accum<KC>(col[j * vbytes], mean[j], U + (j - slot_base) * ldu, kvalid, acc);
   if (!act) return;
-  float* out = part + (int64_t)blockIdx.y * nb * 4 * KC + b * 4 * KC;
+  double* out = part + (int64_t)blockIdx.y * nb * 4 * KC + b * 4 * KC;
 #pragma unroll
   for (int r = 0; r < 4; ++r)
 #pragma unroll
-    for (int c = 0; c < KC; ++c) out[r * KC + c] = acc[r][c];
+    for (int c = 0; c < KC; ++c) out[r * KC + c] = tot[r][c];
 }
 
 // U[local, c0+c] = factor(slot) * sum_split part[split][local][c]
-__global__ void k_reduce_p1(const float* __restrict__ part, int nsplit, int64_t nslots, int KC,
+__global__ void k_reduce_p1(const double* __restrict__ part, int nsplit, int64_t nslots, int KC,
                             int kvalid, int c0, const float* __restrict__ inv, int64_t slot0,
                             CompScale cs, int inv_pow, float* __restrict__ U, int64_t ldu) {
   const int64_t total = nslots * KC;
@@ -141,17 +178,17 @@ __global__ void k_reduce_p1(const float* __restrict__ part, int nsplit, int64_t 
     const int64_t l = idx / KC;
     const int c = static_cast<int>(idx % KC);
     if (c >= kvalid) continue;
-    float sum = 0.0f;
+    double sum = 0.0;
     for (int s = 0; s < nsplit; ++s) sum += part[((int64_t)s * nslots + l) * KC + c];
     const int64_t slot = slot0 + l;
     const float iv = inv[slot];
     const float f = (inv_pow == 2 ? iv * iv : iv) * cs.scale_of(slot);
-    U[l * ldu + c0 + c] = sum * f;
+    U[l * ldu + c0 + c] = static_cast<float>(sum * static_cast<double>(f));
   }
 }
 
 // out_g[i, c0+c] = beta*out_g + sum over g's chunks of part[chunk][i][c]
-__global__ void k_reduce_p2(const float* __restrict__ part, int64_t n, int KC, int kvalid, int c0,
+__global__ void k_reduce_p2(const double* __restrict__ part, int64_t n, int KC, int kvalid, int c0,
                             ChunkTable tab, int nout, int stacked, float* __restrict__ out,
                             int64_t ldo, int64_t out_stride, float beta) {
   const int64_t nb4 = ((n + 3) / 4) * 4;
@@ -167,10 +204,11 @@ __global__ void k_reduce_p2(const float* __restrict__ part, int64_t n, int KC, i
       cb = tab.chunk_off[g];
       ce = tab.chunk_off[g + 1];
     }
-    float sum = 0.0f;
+    double sum = 0.0;
     for (int ch = cb; ch < ce; ++ch) sum += part[((int64_t)ch * nb4 + i) * KC + c];
     float* o = out + (int64_t)g * out_stride + i * ldo + c0 + c;
-    *o = (beta != 0.0f) ? beta * (*o) + sum : sum;
+    const float v = static_cast<float>(sum);
+    *o = (beta != 0.0f) ? beta * (*o) + v : v;
   }
 }
 
@@ -247,9 +285,9 @@ int plan_valu(const coh_panel* p, int64_t k, int32_t cb, int32_t ce, KvPlan* pl)
   pl->ldu = k;
   int64_t off = round_up(nslots * k * 4, 256);
   pl->off_p1 = off;
-  off += round_up((int64_t)pl->nsplit1 * nslots * pl->KC * 4, 256);
+  off += round_up((int64_t)pl->nsplit1 * nslots * pl->KC * 8, 256);
   pl->off_p2 = off;
-  off += round_up((int64_t)pl->tab.nchunks * nb * 4 * pl->KC * 4, 256);
+  off += round_up((int64_t)pl->tab.nchunks * nb * 4 * pl->KC * 8, 256);
   pl->total = off;
   return COH_OK;
 }
@@ -259,7 +297,7 @@ int run_phase1_valu(const coh_panel* p, const KvPlan& pl, const float* V, int64_
   const int64_t nslots = pl.slot1 - pl.slot0;
   const int64_t nbytes1 = nslots / 4;
   float* U = reinterpret_cast<float*>(ws + pl.off_U);
-  float* part = reinterpret_cast<float*>(ws + pl.off_p1);
+  double* part = reinterpret_cast<double*>(ws + pl.off_p1);
   dim3 grid((unsigned)ceil_div(nbytes1, 256), (unsigned)pl.nsplit1);
   for (int64_t c0 = 0; c0 < k; c0 += pl.KC) {
     const int kvalid = static_cast<int>(std::min<int64_t>(pl.KC, k - c0));
@@ -267,6 +305,10 @@ int run_phase1_valu(const coh_panel* p, const KvPlan& pl, const float* V, int64_
 #define COH_P1(KCV)                                                                          \
   hipLaunchKernelGGL(k_p1_valu<KCV>, grid, dim3(256), 0, st, p->smaj, p->sbytes, p->n,        \
                      pl.slot0 / 4, nbytes1, p->mean, Vc, ldv, kvalid, pl.rows_per_split, part)
+    char name1[64];
+    snprintf(name1, sizeof(name1), "k_p1_valu_KC%d", pl.KC);
+    const double fl1 = 2.0 * (double)p->n * (double)pl.m_sel * (double)kvalid;
+    KernelTimer tm1(st, name1, (double)p->n * (double)ceil_div(pl.m_sel, 4), fl1, fl1);
     switch (pl.KC) {
       case 1: COH_P1(1); break;
       case 2: COH_P1(2); break;
@@ -274,6 +316,7 @@ int run_phase1_valu(const coh_panel* p, const KvPlan& pl, const float* V, int64_
       case 8: COH_P1(8); break;
       default: COH_P1(16); break;
     }
+    tm1.stop();
 #undef COH_P1
     COH_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_reduce_p1, dim3(grid_for(nslots * pl.KC)), dim3(256), 0, st, part,
@@ -287,7 +330,7 @@ int run_phase1_valu(const coh_panel* p, const KvPlan& pl, const float* V, int64_
 int run_phase2_valu(const coh_panel* p, const KvPlan& pl, int64_t k, int stacked, float* out,
                     int64_t ldo, int64_t out_stride, float beta, uint8_t* ws, hipStream_t st) {
   const float* U = reinterpret_cast<const float*>(ws + pl.off_U);
-  float* part = reinterpret_cast<float*>(ws + pl.off_p2);
+  double* part = reinterpret_cast<double*>(ws + pl.off_p2);
   const int64_t nb = ceil_div(p->n, 4);
   dim3 grid((unsigned)ceil_div(nb, 256), (unsigned)pl.tab.nchunks);
   const int nout = stacked ? pl.tab.ncomp : 1;
@@ -297,6 +340,10 @@ int run_phase2_valu(const coh_panel* p, const KvPlan& pl, int64_t k, int stacked
 #define COH_P2(KCV)                                                                          \
   hipLaunchKernelGGL(k_p2_valu<KCV>, grid, dim3(256), 0, st, p->vmaj, p->vbytes, p->n, p->mean, \
                      Uc, pl.ldu, pl.slot0, kvalid, pl.tab, part)
+    char name2[64];
+    snprintf(name2, sizeof(name2), "k_p2_valu_KC%d", pl.KC);
+    const double fl2 = 2.0 * (double)p->n * (double)pl.m_sel * (double)kvalid;
+    KernelTimer tm2(st, name2, (double)p->n * (double)ceil_div(pl.m_sel, 4), fl2, fl2);
     switch (pl.KC) {
       case 1: COH_P2(1); break;
       case 2: COH_P2(2); break;
@@ -304,6 +351,7 @@ int run_phase2_valu(const coh_panel* p, const KvPlan& pl, int64_t k, int stacked
       case 8: COH_P2(8); break;
       default: COH_P2(16); break;
     }
+    tm2.stop();
 #undef COH_P2
     COH_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_reduce_p2, dim3(grid_for((int64_t)nout * p->n * pl.KC)), dim3(256), 0, st,

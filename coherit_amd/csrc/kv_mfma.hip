@@ -3,23 +3,33 @@
 // The centred genotype contraction is split into two exact integer
 // contractions: G0 (code value, missing -> 0) and O (observed mask):
 //   phase 1:  T_j   = s_c * ( (G0^T q)_j - mean_j (O^T q)_j )
-//   phase 2:  out_i = s_c * ( (G0 q_u)_i - (O q_w)_i ),  w_j = mean_j u_j
+//   phase 2:  out_i = s_c * ( (G0 q_u)_i + (M q_w)_i ) - W_c,  w_j = mean_j u_j
+// with M the missing mask (O = 1 - M) and W_c = sum_j w_j in FP64 from the
+// unquantised u: only the 1%-dense M carries quantisation noise of w, which
+// matters for rare variants whose centred signal is small
 // where q is the right-hand block quantised per column to fixed point
 // (|q| <= 127*2^16 as three balanced int8 limbs, or 127*2^8 as two) with
-// scale s_c.  Limb products accumulate exactly in int32 on the matrix cores;
-// limbs are combined in FP64 in the epilogue, so the only rounding is the
-// quantisation of q (relative 1.2e-7 of the column max with three limbs).
-// Integer partial sums make the split reductions order-independent.
+// scale s_c (phase 2: one scale per component and column).  Limb products
+// accumulate exactly in int32 on the matrix cores; limbs are combined in FP64
+// in the epilogue, so the only rounding is the quantisation of q (relative
+// 1.2e-7 of the column max with three limbs, 3e-5 with two).  Integer
+// partial sums make the split reductions order-independent.
 //
-// Operand layout: an A-lane (row r = lane%16, K group g = lane/16) loads
-// one dword = 16 consecutive samples (phase 1, variant-major rows) or 16
-// consecutive slots (phase 2, sample-major rows) and decodes it with
-// (w >> 2s) & 0x03030303 followed by a byte LUT (v_perm): register s byte b
-// holds element 16g + 4b + s.  The quantised B operand is stored in the same
-// lane order, so one 16-byte load per lane feeds one MFMA.
+// Operand layout: the contraction axis (samples in phase 1, slots in phase
+// 2) is walked in 256-element super-blocks of four 64-element K-steps.  An
+// A-lane (row r = lane%16, K group g = lane/16) loads 16 bytes = elements
+// 256*sb + 64g .. +64 of its packed row with one dwordx4 load (prefetched
+// one super-block ahead); dword t feeds K-step t and is decoded with
+// (w >> 2s) & 0x03030303 followed by a byte LUT (v_perm), so register s byte
+// b holds element 64g + 16t + 4b + s.  The quantised B operand is stored in
+// that lane order and staged through double-buffered LDS shared by the four
+// waves of a workgroup: the next stage's global loads overlap this stage's
+// MFMAs.
 #include "kv_plan.hpp"
+#include "timing.hpp"
 
 #include <algorithm>
+#include <cstdio>
 
 namespace coh {
 
@@ -27,6 +37,7 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 
 constexpr uint32_t kLutG0 = 0x00020100u;  // code 0,1,2,3 -> 0,1,2,0
 constexpr uint32_t kLutO = 0x00010101u;   // code 0,1,2,3 -> 1,1,1,0
+constexpr uint32_t kLutM = 0x01000000u;   // code 0,1,2,3 -> 0,0,0,1
 constexpr double kQ3 = 127.0 * 65536.0;
 constexpr double kQ2 = 127.0 * 256.0;
 
@@ -36,6 +47,16 @@ __device__ inline void decode16(uint32_t w, v4i& g0, v4i& ob) {
     const uint32_t c = (w >> (2 * s)) & 0x03030303u;
     g0[s] = static_cast<int>(__builtin_amdgcn_perm(0u, kLutG0, c));
     ob[s] = static_cast<int>(__builtin_amdgcn_perm(0u, kLutO, c));
+  }
+}
+
+// Phase-2 planes: G0 and the missing mask M (code 3).
+__device__ inline void decode16m(uint32_t w, v4i& g0, v4i& mb) {
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const uint32_t c = (w >> (2 * s)) & 0x03030303u;
+    g0[s] = static_cast<int>(__builtin_amdgcn_perm(0u, kLutG0, c));
+    mb[s] = static_cast<int>(__builtin_amdgcn_perm(0u, kLutM, c));
   }
 }
 
@@ -65,38 +86,59 @@ __device__ inline int quant(float x, double inv_scale, double qmax) {
 // ---------------------------------------------------------- column scales
 
 // part[blk][c] = max_i |X[i,c]| (and |mean_i X[i,c]| when mean != null)
+// Per-block column max |x| (and |mean*x|); with mean, also the FP64 partial
+// column sums of mean*x (fixed thread and block order: deterministic).
 __global__ __launch_bounds__(256) void k_colmax_part(int64_t rows, int64_t k,
                                                      const float* __restrict__ X, int64_t ldx,
                                                      const float* __restrict__ mean,
-                                                     int64_t mean_off, float* __restrict__ part) {
+                                                     int64_t mean_off, float* __restrict__ part,
+                                                     double* __restrict__ wpart) {
   const int64_t c0 = (int64_t)blockIdx.y * 256;
   const int kk = (int)min<int64_t>(256, k - c0);
   const int R = 256 / kk;
   const int t = threadIdx.x, c = t % kk, r = t / kk;
   float mx = 0.0f;
+  double w = 0.0;
   if (r < R) {
     for (int64_t i = (int64_t)blockIdx.x * R + r; i < rows; i += (int64_t)gridDim.x * R) {
       const float x = X[i * ldx + c0 + c];
       mx = fmaxf(mx, fabsf(x));
-      if (mean) mx = fmaxf(mx, fabsf(mean[mean_off + i] * x));
+      if (mean) {
+        const float mu = mean[mean_off + i];
+        mx = fmaxf(mx, fabsf(mu * x));
+        w += static_cast<double>(mu) * static_cast<double>(x);
+      }
     }
   }
   __shared__ float sh[256];
+  __shared__ double shw[256];
   sh[t] = mx;
+  shw[t] = w;
   __syncthreads();
   if (t < kk) {
     float m = 0.0f;
-    for (int rr = 0; rr < R; ++rr) m = fmaxf(m, sh[rr * kk + t]);
+    double ws = 0.0;
+    for (int rr = 0; rr < R; ++rr) {
+      m = fmaxf(m, sh[rr * kk + t]);
+      ws += shw[rr * kk + t];
+    }
     part[(int64_t)blockIdx.x * k + c0 + t] = m;
+    if (mean) wpart[(int64_t)blockIdx.x * k + c0 + t] = ws;
   }
 }
 
-__global__ void k_colmax_final(const float* __restrict__ part, int nblk, int64_t k, double qmax,
-                               double* __restrict__ scale, double* __restrict__ inv_scale) {
+__global__ void k_colmax_final(const float* __restrict__ part, const double* __restrict__ wpart,
+                               int nblk, int64_t k, double qmax, double* __restrict__ scale,
+                               double* __restrict__ inv_scale, double* __restrict__ wsum) {
   for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < k;
        c += (int64_t)gridDim.x * blockDim.x) {
     float m = 0.0f;
     for (int b = 0; b < nblk; ++b) m = fmaxf(m, part[(int64_t)b * k + c]);
+    if (wsum) {
+      double w = 0.0;
+      for (int b = 0; b < nblk; ++b) w += wpart[(int64_t)b * k + c];
+      wsum[c] = w;
+    }
     // a non-finite column propagates as NaN (its limbs are zero)
     const bool finite = isfinite(m);
     const double s = (m > 0.0f && finite) ? static_cast<double>(m) / qmax : 1.0;
@@ -128,7 +170,7 @@ __global__ __launch_bounds__(256) void k_quant_p1(const float* __restrict__ V, i
     for (int s = 0; s < 4; ++s)
 #pragma unroll
       for (int b = 0; b < 4; ++b) {
-        const int64_t i = kb * 64 + 16 * g + 4 * b + s;
+        const int64_t i = (kb / 4) * 256 + 64 * g + 16 * (kb % 4) + 4 * b + s;
         const float x = (c < k && i < n) ? V[i * ldv + c] : 0.0f;
         int lm[3];
         limbs<L>(quant(x, is, qmax), lm);
@@ -152,7 +194,8 @@ template <int L>
 __global__ __launch_bounds__(256) void k_quant_p2(const float* __restrict__ U, int64_t ldu,
                                                   int64_t nslots, int64_t k, int64_t KB,
                                                   int64_t NBt, const float* __restrict__ mean,
-                                                  int64_t slot0, const double* __restrict__ inv_scale,
+                                                  int64_t slot0, ChunkTable tab,
+                                                  const double* __restrict__ inv_scale,
                                                   double qmax, uint8_t* __restrict__ Bu) {
   const int64_t total = NBt * KB * 64;
   for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
@@ -163,18 +206,22 @@ __global__ __launch_bounds__(256) void k_quant_p2(const float* __restrict__ U, i
     const int cc = lane % 16, g = lane / 16;
     const int64_t c = nb * 16 + cc;
     uint32_t wu[3][4] = {}, ww[3][4] = {};
-    const double is = (c < k) ? inv_scale[c] : 0.0;
+    // components start on 256-slot boundaries: one super-block, one component
+    const int64_t sb_slot = slot0 + (kb / 4) * 256;
+    int comp = 0;
+    while (comp + 1 < tab.ncomp && sb_slot >= tab.end[comp]) ++comp;
+    const double is = (c < k) ? inv_scale[(int64_t)comp * k + c] : 0.0;
 #pragma unroll
     for (int s = 0; s < 4; ++s)
 #pragma unroll
       for (int b = 0; b < 4; ++b) {
-        const int64_t j = kb * 64 + 16 * g + 4 * b + s;
+        const int64_t j = (kb / 4) * 256 + 64 * g + 16 * (kb % 4) + 4 * b + s;
         const bool ok = (c < k) && (j < nslots);
         const float u = ok ? U[j * ldu + c] : 0.0f;
         const float mu = ok ? mean[slot0 + j] : 0.0f;
         int lu[3], lw[3];
         limbs<L>(quant(u, is, qmax), lu);
-        limbs<L>(-quant(mu * u, is, qmax), lw);
+        limbs<L>(quant(mu * u, is, qmax), lw);
 #pragma unroll
         for (int l = 0; l < L; ++l) {
           wu[l][s] |= (static_cast<uint32_t>(lu[l]) & 0xffu) << (8 * b);
@@ -202,8 +249,8 @@ struct P1Args {
   int64_t vbytes;
   int64_t slot0;       // first slot of the range
   int64_t nslots;
-  int64_t KB;          // 64-sample blocks
-  int64_t kb_per_split;
+  int64_t KB;          // 64-sample blocks (a multiple of 4)
+  int64_t sb_per_split;  // 256-sample super-blocks per split
   int64_t NBt;         // 16-column blocks in the quantised B
   int64_t k;
   const uint8_t* Bq;
@@ -217,14 +264,21 @@ struct P1Args {
   int nsplit;
 };
 
-template <int L, int NB, int MB>
+template <int L, int NB, int MB, int T>
 __global__ __launch_bounds__(256) void k_p1_mfma(P1Args a) {
+  // Double-buffered LDS stages of T K-steps of the quantised B block, shared
+  // by the 4 waves; the next stage's global loads overlap this stage's MFMAs.
+  constexpr int STG = L * NB * T * 64;
+  constexpr int PER = (STG + 255) / 256;
+  __shared__ v4i sB[2][STG];
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
   const int r = lane % 16, g = lane / 16;
   const int64_t tile0 = (int64_t)blockIdx.x * (64 * MB) + wave * (16 * MB);
   const int nbg0 = blockIdx.z * NB;  // first 16-col block of this column group
-  const int64_t kb0 = (int64_t)blockIdx.y * a.kb_per_split;
-  const int64_t kb1 = min(a.KB, kb0 + a.kb_per_split);
+  const int64_t nsb = a.KB / 4;
+  const int64_t sb0 = (int64_t)blockIdx.y * a.sb_per_split;
+  const int64_t sb1 = min(nsb, sb0 + a.sb_per_split);
+  const int64_t ks_end = sb1 * 4;
   v4i accg[L][MB][NB], acco[L][MB][NB];
 #pragma unroll
   for (int l = 0; l < L; ++l)
@@ -235,39 +289,81 @@ __global__ __launch_bounds__(256) void k_p1_mfma(P1Args a) {
         accg[l][mb][nb] = v4i{0, 0, 0, 0};
         acco[l][mb][nb] = v4i{0, 0, 0, 0};
       }
+  // lane (r, g) reads 16 bytes = samples 256*sb + 64g .. +64 of its slot row;
+  // dword t of it feeds K-step t (elements 64g + 16t + 4b + s)
   const uint8_t* rowp[MB];
 #pragma unroll
   for (int mb = 0; mb < MB; ++mb)
-    rowp[mb] = a.vmaj + (a.slot0 + tile0 + mb * 16 + r) * a.vbytes + 4 * g;
-  const int64_t bstride_kb = 64 * 16;
-  const int64_t bstride_nb = a.KB * bstride_kb;
-  const int64_t bstride_l = a.NBt * bstride_nb;
-  const uint8_t* bbase = a.Bq + (int64_t)nbg0 * bstride_nb + lane * 16;
-  for (int64_t kb = kb0; kb < kb1; ++kb) {
-    uint32_t w[MB];
+    rowp[mb] = a.vmaj + (a.slot0 + tile0 + mb * 16 + r) * a.vbytes + 16 * g;
+  v4i reg[PER];
+  auto gload = [&](int64_t ks) {
 #pragma unroll
-    for (int mb = 0; mb < MB; ++mb) w[mb] = *reinterpret_cast<const uint32_t*>(rowp[mb] + kb * 16);
-    v4i bq[L][NB];
-#pragma unroll
-    for (int l = 0; l < L; ++l)
-#pragma unroll
-      for (int nb = 0; nb < NB; ++nb)
-        bq[l][nb] = *reinterpret_cast<const v4i*>(bbase + l * bstride_l + nb * bstride_nb +
-                                                   kb * bstride_kb);
-#pragma unroll
-    for (int mb = 0; mb < MB; ++mb) {
-      v4i ag, ao;
-      decode16(w[mb], ag, ao);
-#pragma unroll
-      for (int l = 0; l < L; ++l)
-#pragma unroll
-        for (int nb = 0; nb < NB; ++nb) {
-          accg[l][mb][nb] =
-              __builtin_amdgcn_mfma_i32_16x16x64_i8(ag, bq[l][nb], accg[l][mb][nb], 0, 0, 0);
-          acco[l][mb][nb] =
-              __builtin_amdgcn_mfma_i32_16x16x64_i8(ao, bq[l][nb], acco[l][mb][nb], 0, 0, 0);
-        }
+    for (int p = 0; p < PER; ++p) {
+      const int idx = threadIdx.x + p * 256;
+      if (idx < STG) {
+        const int ln = idx % 64, t = (idx / 64) % T, nb = (idx / (64 * T)) % NB,
+                  l = idx / (64 * T * NB);
+        reg[p] = *reinterpret_cast<const v4i*>(
+            a.Bq + ((((int64_t)l * a.NBt + nbg0 + nb) * a.KB + ks + t) * 64 + ln) * 16);
+      }
     }
+  };
+  auto sstore = [&](int buf) {
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+      const int idx = threadIdx.x + p * 256;
+      if (idx < STG) sB[buf][idx] = reg[p];
+    }
+  };
+  v4i cur[MB], nxt[MB];
+  if (sb0 < sb1) {
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb) cur[mb] = *reinterpret_cast<const v4i*>(rowp[mb] + sb0 * 64);
+    gload(sb0 * 4);
+    sstore(0);
+  }
+  __syncthreads();
+  int buf = 0;
+  for (int64_t sb = sb0; sb < sb1; ++sb) {
+    if (sb + 1 < sb1) {
+#pragma unroll
+      for (int mb = 0; mb < MB; ++mb)
+        nxt[mb] = *reinterpret_cast<const v4i*>(rowp[mb] + (sb + 1) * 64);
+    }
+#pragma unroll
+    for (int st = 0; st < 4 / T; ++st) {
+      const int64_t ks = sb * 4 + st * T;
+      const bool more = ks + T < ks_end;
+      if (more) gload(ks + T);
+#pragma unroll
+      for (int tt = 0; tt < T; ++tt) {
+        const int t = st * T + tt;
+        v4i b[L][NB];
+#pragma unroll
+        for (int l = 0; l < L; ++l)
+#pragma unroll
+          for (int nb = 0; nb < NB; ++nb) b[l][nb] = sB[buf][((l * NB + nb) * T + tt) * 64 + lane];
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb) {
+          v4i ag, ao;
+          decode16(static_cast<uint32_t>(cur[mb][t]), ag, ao);
+#pragma unroll
+          for (int l = 0; l < L; ++l)
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb) {
+              accg[l][mb][nb] =
+                  __builtin_amdgcn_mfma_i32_16x16x64_i8(ag, b[l][nb], accg[l][mb][nb], 0, 0, 0);
+              acco[l][mb][nb] =
+                  __builtin_amdgcn_mfma_i32_16x16x64_i8(ao, b[l][nb], acco[l][mb][nb], 0, 0, 0);
+            }
+        }
+      }
+      if (more) sstore(buf ^ 1);
+      __syncthreads();
+      buf ^= 1;
+    }
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb) cur[mb] = nxt[mb];
   }
   // C layout: lane holds column (lane % 16), rows 4*(lane/16) + v.
 #pragma unroll
@@ -289,8 +385,8 @@ __global__ __launch_bounds__(256) void k_p1_mfma(P1Args a) {
           const int64_t slot = a.slot0 + jl;
           const double iv = a.inv[slot];
           const double f = (a.inv_pow == 2 ? iv * iv : iv) * a.cs.scale_of(slot);
-          const double t = a.scale[col] * (A1 - static_cast<double>(a.mean[slot]) * A2);
-          a.U[jl * a.k + col] = static_cast<float>(f * t);
+          const double tv = a.scale[col] * (A1 - static_cast<double>(a.mean[slot]) * A2);
+          a.U[jl * a.k + col] = static_cast<float>(f * tv);
         } else {
           int32_t* dst = a.part + (((int64_t)blockIdx.y * a.nslots + jl) * a.k + col) * (2 * L);
 #pragma unroll
@@ -302,6 +398,7 @@ __global__ __launch_bounds__(256) void k_p1_mfma(P1Args a) {
       }
     }
 }
+
 
 template <int L>
 __global__ void k_p1_reduce(P1Args a) {
@@ -348,15 +445,20 @@ struct P2Args {
   int32_t* part;       // [chunk][n][k][L]
 };
 
-template <int L, int NB, int MB>
+template <int L, int NB, int MB, int T>
 __global__ __launch_bounds__(256) void k_p2_mfma(P2Args a) {
+  // Double-buffered LDS stages of T K-steps (u and mean*u halves).
+  constexpr int STG = L * NB * T * 2 * 64;
+  constexpr int PER = (STG + 255) / 256;
+  __shared__ v4i sB[2][STG];
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
   const int r = lane % 16, g = lane / 16;
   const int64_t i0 = (int64_t)blockIdx.x * (64 * MB) + wave * (16 * MB);
   const int nbg0 = blockIdx.z * NB;
   int64_t j0, j1;
   a.tab.chunk_range(blockIdx.y, &j0, &j1);
-  const int64_t kb0 = (j0 - a.slot0) / 64, kb1 = (j1 - a.slot0) / 64;
+  const int64_t sb0 = (j0 - a.slot0) / 256, sb1 = (j1 - a.slot0) / 256;
+  const int64_t ks_end = sb1 * 4;
   v4i acc[L][MB][NB];
 #pragma unroll
   for (int l = 0; l < L; ++l)
@@ -370,40 +472,85 @@ __global__ __launch_bounds__(256) void k_p2_mfma(P2Args a) {
   for (int mb = 0; mb < MB; ++mb) {
     const int64_t i = i0 + mb * 16 + r;
     rowok[mb] = i < a.n;
-    rowp[mb] = a.smaj + (rowok[mb] ? i : 0) * a.sbytes + a.slot0 / 4 + 4 * g;
+    rowp[mb] = a.smaj + (rowok[mb] ? i : 0) * a.sbytes + a.slot0 / 4 + 16 * g;
   }
-  const int64_t bstride_kb = 2 * 64 * 16;
-  const int64_t bstride_nb = a.KB * bstride_kb;
-  const int64_t bstride_l = a.NBt * bstride_nb;
-  const uint8_t* bbase = a.Bu + (int64_t)nbg0 * bstride_nb + lane * 16;
-  for (int64_t kb = kb0; kb < kb1; ++kb) {
-    uint32_t w[MB];
+  v4i reg[PER];
+  auto gload = [&](int64_t ks) {
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+      const int idx = threadIdx.x + p * 256;
+      if (idx < STG) {
+        const int ln = idx % 64, h = (idx / 64) % 2, t = (idx / 128) % T,
+                  nb = (idx / (128 * T)) % NB, l = idx / (128 * T * NB);
+        reg[p] = *reinterpret_cast<const v4i*>(
+            a.Bu + (((((int64_t)l * a.NBt + nbg0 + nb) * a.KB + ks + t) * 2 + h) * 64 + ln) * 16);
+      }
+    }
+  };
+  auto sstore = [&](int buf) {
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+      const int idx = threadIdx.x + p * 256;
+      if (idx < STG) sB[buf][idx] = reg[p];
+    }
+  };
+  v4i cur[MB], nxt[MB];
+#pragma unroll
+  for (int mb = 0; mb < MB; ++mb) cur[mb] = v4i{0, 0, 0, 0};
+  if (sb0 < sb1) {
 #pragma unroll
     for (int mb = 0; mb < MB; ++mb)
-      w[mb] = rowok[mb] ? *reinterpret_cast<const uint32_t*>(rowp[mb] + kb * 16) : 0u;
-    v4i bu[L][NB], bw[L][NB];
+      if (rowok[mb]) cur[mb] = *reinterpret_cast<const v4i*>(rowp[mb] + sb0 * 64);
+    gload(sb0 * 4);
+    sstore(0);
+  }
+  __syncthreads();
+  int buf = 0;
+  for (int64_t sb = sb0; sb < sb1; ++sb) {
+    if (sb + 1 < sb1) {
 #pragma unroll
-    for (int l = 0; l < L; ++l)
-#pragma unroll
-      for (int nb = 0; nb < NB; ++nb) {
-        const uint8_t* p = bbase + l * bstride_l + nb * bstride_nb + kb * bstride_kb;
-        bu[l][nb] = *reinterpret_cast<const v4i*>(p);
-        bw[l][nb] = *reinterpret_cast<const v4i*>(p + 64 * 16);
-      }
-#pragma unroll
-    for (int mb = 0; mb < MB; ++mb) {
-      v4i ag, ao;
-      decode16(w[mb], ag, ao);
-#pragma unroll
-      for (int l = 0; l < L; ++l)
-#pragma unroll
-        for (int nb = 0; nb < NB; ++nb) {
-          acc[l][mb][nb] =
-              __builtin_amdgcn_mfma_i32_16x16x64_i8(ag, bu[l][nb], acc[l][mb][nb], 0, 0, 0);
-          acc[l][mb][nb] =
-              __builtin_amdgcn_mfma_i32_16x16x64_i8(ao, bw[l][nb], acc[l][mb][nb], 0, 0, 0);
-        }
+      for (int mb = 0; mb < MB; ++mb)
+        nxt[mb] = rowok[mb] ? *reinterpret_cast<const v4i*>(rowp[mb] + (sb + 1) * 64)
+                            : v4i{0, 0, 0, 0};
     }
+#pragma unroll
+    for (int st = 0; st < 4 / T; ++st) {
+      const int64_t ks = sb * 4 + st * T;
+      const bool more = ks + T < ks_end;
+      if (more) gload(ks + T);
+#pragma unroll
+      for (int tt = 0; tt < T; ++tt) {
+        const int t = st * T + tt;
+        v4i bu[L][NB], bw[L][NB];
+#pragma unroll
+        for (int l = 0; l < L; ++l)
+#pragma unroll
+          for (int nb = 0; nb < NB; ++nb) {
+            const int base = (((l * NB + nb) * T + tt) * 2) * 64 + lane;
+            bu[l][nb] = sB[buf][base];
+            bw[l][nb] = sB[buf][base + 64];
+          }
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb) {
+          v4i ag, am;
+          decode16m(static_cast<uint32_t>(cur[mb][t]), ag, am);
+#pragma unroll
+          for (int l = 0; l < L; ++l)
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb) {
+              acc[l][mb][nb] =
+                  __builtin_amdgcn_mfma_i32_16x16x64_i8(ag, bu[l][nb], acc[l][mb][nb], 0, 0, 0);
+              acc[l][mb][nb] =
+                  __builtin_amdgcn_mfma_i32_16x16x64_i8(am, bw[l][nb], acc[l][mb][nb], 0, 0, 0);
+            }
+        }
+      }
+      if (more) sstore(buf ^ 1);
+      __syncthreads();
+      buf ^= 1;
+    }
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb) cur[mb] = nxt[mb];
   }
 #pragma unroll
   for (int mb = 0; mb < MB; ++mb)
@@ -422,9 +569,11 @@ __global__ __launch_bounds__(256) void k_p2_mfma(P2Args a) {
     }
 }
 
+
 template <int L>
 __global__ void k_p2_reduce(const int32_t* __restrict__ part, int64_t n, int64_t k,
-                            const double* __restrict__ scale, ChunkTable tab, int nout, int stacked,
+                            const double* __restrict__ scale, const double* __restrict__ wsum,
+                            ChunkTable tab, int nout, int stacked,
                             float* __restrict__ out, int64_t ldo, int64_t out_stride, float beta) {
   const int64_t total = (int64_t)nout * n * k;
   for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
@@ -432,21 +581,22 @@ __global__ void k_p2_reduce(const int32_t* __restrict__ part, int64_t n, int64_t
     const int64_t col = idx % k;
     const int64_t i = (idx / k) % n;
     const int go = static_cast<int>(idx / (k * n));
-    int cb = 0, ce = tab.nchunks;
-    if (stacked) {
-      cb = tab.chunk_off[go];
-      ce = tab.chunk_off[go + 1];
-    }
-    int64_t s[3] = {0, 0, 0};
-    for (int ch = cb; ch < ce; ++ch) {
-      const int32_t* src = part + (((int64_t)ch * n + i) * k + col) * L;
+    const int g0 = stacked ? go : 0;
+    const int g1 = stacked ? go + 1 : tab.ncomp;
+    double acc = 0.0;
+    for (int g = g0; g < g1; ++g) {
+      int64_t s[3] = {0, 0, 0};
+      for (int ch = tab.chunk_off[g]; ch < tab.chunk_off[g + 1]; ++ch) {
+        const int32_t* src = part + (((int64_t)ch * n + i) * k + col) * L;
 #pragma unroll
-      for (int l = 0; l < L; ++l) s[l] += src[l];
-    }
-    double t = 0.0;
+        for (int l = 0; l < L; ++l) s[l] += src[l];
+      }
+      double t = 0.0;
 #pragma unroll
-    for (int l = 0; l < L; ++l) t += ldexp(static_cast<double>(s[l]), 8 * l);
-    const float v = static_cast<float>(scale[col] * t);
+      for (int l = 0; l < L; ++l) t += ldexp(static_cast<double>(s[l]), 8 * l);
+      acc += scale[(int64_t)g * k + col] * t - wsum[(int64_t)g * k + col];
+    }
+    const float v = static_cast<float>(acc);
     float* o = out + (int64_t)go * out_stride + i * ldo + col;
     *o = (beta != 0.0f) ? beta * (*o) + v : v;
   }
@@ -479,15 +629,16 @@ int plan_mfma(const coh_panel* p, int precision, int64_t k, int32_t cb, int32_t 
   pl->KC = sh.NB * 16;
   const int64_t groups = ceil_div(ceil_div(k, 16), sh.NB);
   const int64_t NBt = groups * sh.NB;  // column blocks padded to whole groups (zero-filled)
-  const int64_t KB1 = ceil_div(p->n, 64);
-  pl->n_pad = KB1 * 64;
+  const int64_t NSB = ceil_div(p->n, 256);  // 256-sample super-blocks
+  const int64_t KB1 = NSB * 4;
+  pl->n_pad = NSB * 256;
   // phase-1 sample splits only when there are too few slot tiles
   const int64_t tiles1 = std::max<int64_t>(1, nslots / (64 * sh.MB1));
   int64_t nsplit = std::max<int64_t>(1, ceil_div(kTargetBlocks, tiles1 * groups));
-  nsplit = std::min<int64_t>(nsplit, std::max<int64_t>(1, KB1 / 8));
+  nsplit = std::min<int64_t>(nsplit, std::max<int64_t>(1, NSB / 2));
   nsplit = std::min<int64_t>(nsplit, 64);
-  pl->nsplit1 = static_cast<int>(nsplit);
-  pl->rows_per_split = ceil_div(KB1, nsplit);  // in 64-sample blocks
+  pl->rows_per_split = ceil_div(NSB, nsplit);  // in super-blocks
+  pl->nsplit1 = static_cast<int>(ceil_div(NSB, pl->rows_per_split));
   // phase-2 slot chunks
   const int64_t xb2 = std::max<int64_t>(1, ceil_div(p->n, 64 * sh.MB2)) * groups;
   const int64_t target_chunks = std::max<int64_t>(1, ceil_div(kTargetBlocks, xb2));
@@ -502,8 +653,10 @@ int plan_mfma(const coh_panel* p, int precision, int64_t k, int32_t cb, int32_t 
   off += round_up((int64_t)L * NBt * KB1 * 64 * 16, 256);
   pl->off_p2 = off;  // Bu
   off += round_up((int64_t)L * NBt * (nslots / 64) * 2 * 64 * 16, 256);
-  pl->off_aux = off;  // scales: 2 x k doubles, colmax partials
-  off += round_up(2 * k * 8, 256) + round_up((int64_t)kColmaxBlocks * k * 4, 256);
+  pl->off_aux = off;  // scales and inverse scales per (component, column), colmax partials
+  off += round_up(3 * (int64_t)std::max(1, pl->tab.ncomp) * k * 8, 256) +
+         round_up((int64_t)kColmaxBlocks * k * 4, 256) +
+         round_up((int64_t)kColmaxBlocks * k * 8, 256);
   pl->off_p1 = off;
   int64_t p1 = (nsplit > 1) ? nsplit * nslots * k * 2 * L * 4 : 0;
   int64_t p2 = (int64_t)pl->tab.nchunks * p->n * k * L * 4;
@@ -512,17 +665,42 @@ int plan_mfma(const coh_panel* p, int precision, int64_t k, int32_t cb, int32_t 
   return COH_OK;
 }
 
+// Per (component g, column c) at aux: scale[g * k + c], inv_scale and the
+// phase-2 mean term wsum at the same index in the next two ncomp * k blocks
+// (ncomp = components of the plan, at least 1), then colmax partials.
+struct ScaleView {
+  double* scale;
+  double* inv_scale;
+  double* wsum;
+  float* part;
+  double* wpart;
+};
+
+static ScaleView scale_view(const KvPlan& pl, int64_t k, uint8_t* ws) {
+  uint8_t* aux = ws + pl.off_aux;
+  const int64_t nc = std::max(1, pl.tab.ncomp);
+  ScaleView v;
+  v.scale = reinterpret_cast<double*>(aux);
+  v.inv_scale = v.scale + nc * k;
+  v.wsum = v.inv_scale + nc * k;
+  uint8_t* p = aux + round_up(3 * nc * k * 8, 256);
+  v.part = reinterpret_cast<float*>(p);
+  v.wpart = reinterpret_cast<double*>(p + round_up((int64_t)kColmaxBlocks * k * 4, 256));
+  return v;
+}
+
+// Column max |x| (and |mean * x| when mean is given) of rows [0, rows) of X
+// -> scale[c] = max / qmax, inv_scale[c] = 1 / scale[c]; with mean, also
+// wsum[c] = sum_i mean_i x_ic in FP64.
 static int colscale(const float* X, int64_t rows, int64_t k, int64_t ldx, const float* mean,
-                    int64_t mean_off, double qmax, uint8_t* aux, hipStream_t st) {
-  double* scale = reinterpret_cast<double*>(aux);
-  double* inv_scale = scale + k;
-  float* part = reinterpret_cast<float*>(aux + round_up(2 * k * 8, 256));
+                    int64_t mean_off, double qmax, double* scale, double* inv_scale, double* wsum,
+                    float* part, double* wpart, hipStream_t st) {
   const int nblk = (int)std::max<int64_t>(1, std::min<int64_t>(kColmaxBlocks, ceil_div(rows * std::min<int64_t>(k, 256), 4096)));
   hipLaunchKernelGGL(k_colmax_part, dim3(nblk, (unsigned)ceil_div(k, 256)), dim3(256), 0, st, rows,
-                     k, X, ldx, mean, mean_off, part);
+                     k, X, ldx, mean, mean_off, part, wpart);
   COH_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_colmax_final, dim3((unsigned)ceil_div(k, 256)), dim3(256), 0, st, part, nblk,
-                     k, qmax, scale, inv_scale);
+  hipLaunchKernelGGL(k_colmax_final, dim3((unsigned)ceil_div(k, 256)), dim3(256), 0, st, part,
+                     wpart, nblk, k, qmax, scale, inv_scale, mean ? wsum : nullptr);
   COH_CHECK_LAUNCH();
   return COH_OK;
 }
@@ -536,11 +714,12 @@ static int phase1(const coh_panel* p, const KvPlan& pl, const float* V, int64_t 
   const int64_t groups = ceil_div(ceil_div(k, 16), sh.NB);
   const int64_t NBt = groups * sh.NB;
   const int64_t KB = pl.n_pad / 64;
-  uint8_t* aux = ws + pl.off_aux;
-  int s = colscale(V, p->n, k, ldv, nullptr, 0, qmax, aux, st);
+  const ScaleView sv = scale_view(pl, k, ws);
+  int s = colscale(V, p->n, k, ldv, nullptr, 0, qmax, sv.scale, sv.inv_scale, nullptr, sv.part,
+                   nullptr, st);
   if (s != COH_OK) return s;
-  const double* scale = reinterpret_cast<const double*>(aux);
-  const double* inv_scale = scale + k;
+  const double* scale = sv.scale;
+  const double* inv_scale = sv.inv_scale;
   uint8_t* Bq = ws + pl.off_q;
   hipLaunchKernelGGL(k_quant_p1<L>, dim3(grid1d(NBt * KB * 64)), dim3(256), 0, st, V, ldv, p->n,
                      k, KB, NBt, inv_scale, qmax, Bq);
@@ -551,7 +730,7 @@ static int phase1(const coh_panel* p, const KvPlan& pl, const float* V, int64_t 
   a.slot0 = pl.slot0;
   a.nslots = nslots;
   a.KB = KB;
-  a.kb_per_split = pl.rows_per_split;
+  a.sb_per_split = pl.rows_per_split;
   a.NBt = NBt;
   a.k = k;
   a.Bq = Bq;
@@ -564,12 +743,18 @@ static int phase1(const coh_panel* p, const KvPlan& pl, const float* V, int64_t 
   a.part = reinterpret_cast<int32_t*>(ws + pl.off_p1);
   a.nsplit = pl.nsplit1;
   dim3 grid((unsigned)(nslots / (64 * sh.MB1)), (unsigned)pl.nsplit1, (unsigned)groups);
+  char name[64];
+  snprintf(name, sizeof(name), "k_p1_mfma_L%d_NB%d", L, sh.NB);
+  KernelTimer tm(st, name, (double)p->n * (double)ceil_div(pl.m_sel, 4),
+                 2.0 * (double)p->n * (double)pl.m_sel * (double)k,
+                 2.0 * L * 2.0 * (double)nslots * (double)pl.n_pad * (double)(NBt * 16));
   if (sh.NB == 1)
-    hipLaunchKernelGGL((k_p1_mfma<L, 1, 4>), grid, dim3(256), 0, st, a);
+    hipLaunchKernelGGL((k_p1_mfma<L, 1, 4, 4>), grid, dim3(256), 0, st, a);
   else if (sh.NB == 2)
-    hipLaunchKernelGGL((k_p1_mfma<L, 2, 2>), grid, dim3(256), 0, st, a);
+    hipLaunchKernelGGL((k_p1_mfma<L, 2, 2, 1>), grid, dim3(256), 0, st, a);
   else
-    hipLaunchKernelGGL((k_p1_mfma<L, 4, 1>), grid, dim3(256), 0, st, a);
+    hipLaunchKernelGGL((k_p1_mfma<L, 4, 1, 1>), grid, dim3(256), 0, st, a);
+  tm.stop();
   COH_CHECK_LAUNCH();
   if (pl.nsplit1 > 1) {
     hipLaunchKernelGGL(k_p1_reduce<L>, dim3(grid1d(nslots * k)), dim3(256), 0, st, a);
@@ -588,14 +773,23 @@ static int phase2(const coh_panel* p, const KvPlan& pl, int64_t k, int stacked, 
   const int64_t NBt = groups * sh.NB;
   const int64_t KB = nslots / 64;
   const float* U = reinterpret_cast<const float*>(ws + pl.off_U);
-  uint8_t* aux = ws + pl.off_aux;
-  int s = colscale(U, nslots, k, pl.ldu, p->mean, pl.slot0, qmax, aux, st);
-  if (s != COH_OK) return s;
-  const double* scale = reinterpret_cast<const double*>(aux);
-  const double* inv_scale = scale + k;
+  // one fixed-point scale per (component, column): the rows of U carry each
+  // component's factor (1/eff_g, theta_g/eff_g), so a shared scale would
+  // spend the limbs' range on the largest component
+  const ScaleView sv = scale_view(pl, k, ws);
+  for (int g = 0; g < pl.tab.ncomp; ++g) {
+    const int64_t r0 = pl.tab.begin[g] - pl.slot0;
+    const int64_t rows = pl.tab.end[g] - pl.tab.begin[g];
+    int s = colscale(U + r0 * pl.ldu, rows, k, pl.ldu, p->mean, pl.tab.begin[g], qmax,
+                     sv.scale + g * k, sv.inv_scale + g * k, sv.wsum + g * k, sv.part, sv.wpart,
+                     st);
+    if (s != COH_OK) return s;
+  }
+  const double* scale = sv.scale;
+  const double* inv_scale = sv.inv_scale;
   uint8_t* Bu = ws + pl.off_p2;
   hipLaunchKernelGGL(k_quant_p2<L>, dim3(grid1d(NBt * KB * 64)), dim3(256), 0, st, U, pl.ldu,
-                     nslots, k, KB, NBt, p->mean, pl.slot0, inv_scale, qmax, Bu);
+                     nslots, k, KB, NBt, p->mean, pl.slot0, pl.tab, inv_scale, qmax, Bu);
   COH_CHECK_LAUNCH();
   P2Args a;
   a.smaj = p->smaj;
@@ -610,16 +804,24 @@ static int phase2(const coh_panel* p, const KvPlan& pl, int64_t k, int stacked, 
   a.tab = pl.tab;
   a.part = reinterpret_cast<int32_t*>(ws + pl.off_p1);
   dim3 grid((unsigned)ceil_div(p->n, 64 * sh.MB2), (unsigned)pl.tab.nchunks, (unsigned)groups);
+  char name[64];
+  snprintf(name, sizeof(name), "k_p2_mfma_L%d_NB%d", L, sh.NB);
+  KernelTimer tm(st, name, (double)p->n * (double)ceil_div(pl.m_sel, 4),
+                 2.0 * (double)p->n * (double)pl.m_sel * (double)k,
+                 2.0 * L * 2.0 * (double)nslots * (double)round_up(p->n, 64 * sh.MB2) *
+                     (double)(NBt * 16));
   if (sh.NB == 1)
-    hipLaunchKernelGGL((k_p2_mfma<L, 1, 8>), grid, dim3(256), 0, st, a);
+    hipLaunchKernelGGL((k_p2_mfma<L, 1, 8, 4>), grid, dim3(256), 0, st, a);
   else if (sh.NB == 2)
-    hipLaunchKernelGGL((k_p2_mfma<L, 2, 4>), grid, dim3(256), 0, st, a);
+    hipLaunchKernelGGL((k_p2_mfma<L, 2, 4, 1>), grid, dim3(256), 0, st, a);
   else
-    hipLaunchKernelGGL((k_p2_mfma<L, 4, 2>), grid, dim3(256), 0, st, a);
+    hipLaunchKernelGGL((k_p2_mfma<L, 4, 2, 1>), grid, dim3(256), 0, st, a);
+  tm.stop();
   COH_CHECK_LAUNCH();
   const int nout = stacked ? pl.tab.ncomp : 1;
   hipLaunchKernelGGL(k_p2_reduce<L>, dim3(grid1d((int64_t)nout * p->n * k)), dim3(256), 0, st,
-                     a.part, p->n, k, scale, pl.tab, nout, stacked, out, ldo, out_stride, beta);
+                     a.part, p->n, k, scale, sv.wsum, pl.tab, nout, stacked, out, ldo,
+                     out_stride, beta);
   COH_CHECK_LAUNCH();
   return COH_OK;
 }

@@ -51,7 +51,15 @@ struct KvPlan {
   // MFMA plan fields
   int limbs;
   int64_t n_pad;            // samples padded to the MFMA K tile
+  int64_t m_sel;            // real SNPs of the requested components
 };
+
+inline int64_t selected_snps(const coh_panel* p, int32_t cb, int32_t ce) {
+  int64_t m = 0;
+  for (int32_t c = 0; c < p->n_calls; ++c)
+    if (p->call_comp[c] >= cb && p->call_comp[c] < ce) m += p->call_width[c];
+  return m;
+}
 
 void build_chunks(const coh_panel* p, int32_t cb, int32_t ce, int64_t chunk_len, ChunkTable* t);
 int plan_valu(const coh_panel* p, int64_t k, int32_t cb, int32_t ce, KvPlan* pl);
@@ -70,6 +78,7 @@ int run_phase2_mfma(const coh_panel* p, const KvPlan& pl, int64_t k, int stacked
 inline int make_plan(const coh_panel* p, int precision, int64_t k, int32_t cb, int32_t ce,
                      KvPlan* pl) {
   pl->precision = precision;
+  pl->m_sel = selected_snps(p, cb, ce);
   if (precision == COH_PREC_FP32) return plan_valu(p, k, cb, ce, pl);
   if (precision == COH_PREC_I8X3 || precision == COH_PREC_I8X2)
     return plan_mfma(p, precision, k, cb, ce, pl);

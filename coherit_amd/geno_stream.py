@@ -405,6 +405,13 @@ class GenoBlockStreamer:
         self.precision = precision
         self._prec = N.PRECISIONS[precision]
 
+    @property
+    def residual_precision(self) -> str:
+        """Kernels for PCG true residuals: the 2-limb form (matvec error ~1e-4
+        relative) is refined against the 3-limb form (~1e-7); the others are
+        accurate enough to verify themselves."""
+        return "i8x3" if self.precision == "i8x2" else self.precision
+
     def _check_component(self, component_idx):
         if component_idx < 0 or component_idx >= self._n_components:
             raise IndexError(
@@ -446,8 +453,12 @@ class GenoBlockStreamer:
         out = self._panel.kv(V, scale, 0, self._n_components, stacked=True, precision=self._prec)
         return out[:, :, 0] if sq else out
 
-    def weighted_component_hv(self, theta_g, theta_e, V):
-        """theta_e V + sum_g theta_g/eff_g Z_g Z_g^T V (kv_impl.py:1086-1133)."""
+    def weighted_component_hv(self, theta_g, theta_e, V, precision: str | None = None):
+        """theta_e V + sum_g theta_g/eff_g Z_g Z_g^T V (kv_impl.py:1086-1133).
+
+        ``precision`` overrides the streamer's kernels for this call only.
+        """
+        prec = self._prec if precision is None else N.PRECISIONS[precision]
         th = _host_vector(theta_g)
         if th.size != self._n_components:
             raise ValueError(
@@ -459,11 +470,11 @@ class GenoBlockStreamer:
         effs = self._component_eff_m_host
         scale = np.asarray([t / e if e > 0 else 0.0 for t, e in zip(th, effs)], np.float32)
         if theta_e is None:
-            out = self._panel.kv(V, scale, 0, self._n_components, precision=self._prec)
+            out = self._panel.kv(V, scale, 0, self._n_components, precision=prec)
         else:
             te = float(_host_vector(theta_e).reshape(-1)[0])
             out = (V * te).contiguous()
-            self._panel.kv(V, scale, 0, self._n_components, out=out, beta=1.0, precision=self._prec)
+            self._panel.kv(V, scale, 0, self._n_components, out=out, beta=1.0, precision=prec)
         return out[:, 0] if sq else out
 
     def xtv(self, V, normalize: bool = False):

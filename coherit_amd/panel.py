@@ -14,6 +14,16 @@ import torch
 from . import _native as N
 
 
+# When set to a list, every operator call appends (op, k, n, m_sel, precision):
+# the genotype passes of a workload, e.g. to extrapolate a CPU baseline.
+CALL_LOG: list | None = None
+
+
+def _log(op, k, n, m_sel, precision):
+    if CALL_LOG is not None:
+        CALL_LOG.append((op, int(k), int(n), int(m_sel), int(precision)))
+
+
 def _dev(device) -> torch.device:
     if device is None or device == "auto":
         return torch.device("cuda", torch.cuda.current_device())
@@ -131,6 +141,10 @@ class ResidentPanel:
         return out
 
     # ----------------------------------------------------------- operators
+    def _m_sel(self, cb, ce):
+        sel = (self.call_components >= cb) & (self.call_components < ce)
+        return int(self.call_widths[sel].astype(np.int64).sum())
+
     def workspace_bytes(self, op: int, precision: int, k: int, cb: int, ce: int) -> int:
         nbytes = ctypes.c_uint64()
         N.call("coh_workspace_bytes", self.handle, int(op), int(precision), int(k), int(cb),
@@ -160,6 +174,7 @@ class ResidentPanel:
             ldo, ostride = int(out.stride(1)), int(out.stride(0))
         else:
             ldo, ostride = int(out.stride(0)), 0
+        _log("kv", k, self.n, self._m_sel(cb, ce), precision)
         N.call("coh_kv", self.handle, int(precision), N.ptr(V), k, int(V.stride(0)), int(cb), int(ce),
                N.ptr(scale), int(bool(stacked)), N.ptr(out), ldo, ostride, float(beta), N.ptr(ws),
                nb, N.stream_ptr())
@@ -176,6 +191,7 @@ class ResidentPanel:
             rows = int(self.call_widths[sel].astype(np.int64).sum())
         out = torch.empty((rows, k), dtype=torch.float32, device=self.device)
         ws, nb = self._workspace(N.OP_XTV, precision, k, cb, ce)
+        _log("xtv", k, self.n, self._m_sel(cb, ce), precision)
         N.call("coh_xtv", self.handle, int(precision), N.ptr(V), k, int(V.stride(0)), int(cb),
                int(ce), float(scale), int(bool(slot_space)), N.ptr(out), int(out.stride(0)),
                N.ptr(ws), nb, N.stream_ptr())
@@ -202,6 +218,7 @@ class ResidentPanel:
             beta = 0.0
         ws, nb = self._workspace(N.OP_KV, precision, k, cb, ce)
         ldo, ostride = (int(out.stride(1)), int(out.stride(0))) if stacked else (int(out.stride(0)), 0)
+        _log("zb", k, self.n, m_sel, precision)
         N.call("coh_zb", self.handle, int(precision), N.ptr(B), k, int(B.stride(0)), int(cb), int(ce),
                int(bool(stacked)), N.ptr(out), ldo, ostride, float(beta), N.ptr(ws), nb,
                N.stream_ptr())

@@ -86,15 +86,24 @@ def pcg_solve(
     maxiter: int = 200,
     X0=None,
     check_every: int = 2,
+    Hv_true: Optional[Callable] = None,
 ):
     """Solve H X = B with block PCG; returns ``(X, max_rel_res, iters)``.
 
     ``max_rel_res`` is the true relative residual ``max_c ||B - H X||/||B||``
     of the returned X (pcg.py:86-97).  Raises FloatingPointError on a
     non-finite residual or breakdown (pcg.py:118-119, 151-154).
+
+    ``Hv_true`` (default ``Hv``) computes the true residuals that decide
+    convergence and seed each restart.  With a cheaper, less accurate ``Hv``
+    driving the recurrence this is iterative refinement: every restart
+    solves for the correction of the accurate residual, so the returned X
+    meets ``tol`` against ``Hv_true`` even below ``Hv``'s own error floor.
     """
     if M is None:
         M = _identity
+    if Hv_true is None:
+        Hv_true = Hv
     check_every = max(1, int(check_every))
     B = torch.as_tensor(B)
     squeeze = B.ndim == 1
@@ -127,7 +136,7 @@ def pcg_solve(
         solution = torch.where(nonzero.to(torch.bool)[None, :], X0t, torch.zeros_like(X0t))
 
     def true_residual(value):
-        residual = _f32((B2 - op(Hv, value)) / scale32)
+        residual = _f32((B2 - op(Hv_true, value)) / scale32)
         norms = torch.sqrt(_col_dots(residual, residual))
         relative = float((norms / b_norm).max())
         if not math.isfinite(relative):

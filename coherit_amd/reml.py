@@ -87,6 +87,8 @@ class REMLContext:
     slq_reference_runtime: object = None
     slq_workspace_bytes: int = 256 * 1024**2
     counters: dict | None = None
+    # accurate H.V for PCG true residuals when weighted_hv runs a cheaper form
+    weighted_hv_true: Optional[Callable] = None
 
 
 @dataclass
@@ -405,6 +407,13 @@ def _eval_once(ctx, pvec, warm_all, warm_ai=None, *, key_slq, minq_tol, maxiter,
             return ctx.weighted_hv(theta_g, te, V)
         return _hv_apply(ctx.K_mvs, theta_g, theta_e, V, ctx.residual_diag_stack)
 
+    Hv_true = None
+    if ctx.weighted_hv_true is not None:
+        def Hv_true(V):
+            counters["hv_true"] = counters.get("hv_true", 0) + 1
+            te = theta_e[0] if ctx.E == 1 else theta_e
+            return ctx.weighted_hv_true(theta_g, te, V)
+
     diag_H = scalar_diag_from_precond_conf(ctx.precond_conf, theta_g, theta_e)
     if diag_H is None:
         if ctx.diag_atoms is not None:
@@ -431,7 +440,8 @@ def _eval_once(ctx, pvec, warm_all, warm_ai=None, *, key_slq, minq_tol, maxiter,
         X0 = M_cur(rhs_all)
     else:
         X0 = torch.zeros_like(rhs_all)
-    sol_all, main_rel, k_pcg = pcg_solve(Hv, rhs_all, M=M_cur, tol=minq_tol, maxiter=maxiter, X0=X0)
+    sol_all, main_rel, k_pcg = pcg_solve(Hv, rhs_all, M=M_cur, tol=minq_tol, maxiter=maxiter, X0=X0,
+                                         Hv_true=Hv_true)
     del X0
     if not math.isfinite(main_rel) or main_rel > minq_tol * 1.05:
         raise FloatingPointError(
@@ -513,7 +523,8 @@ def _eval_once(ctx, pvec, warm_all, warm_ai=None, *, key_slq, minq_tol, maxiter,
     else:
         X0a = torch.zeros_like(GPy_cols)
     t0 = time.perf_counter()
-    HinvGPy, ai_rel, k_ai = pcg_solve(Hv, GPy_cols, M=M_cur, tol=minq_tol, maxiter=maxiter, X0=X0a)
+    HinvGPy, ai_rel, k_ai = pcg_solve(Hv, GPy_cols, M=M_cur, tol=minq_tol, maxiter=maxiter, X0=X0a,
+                                      Hv_true=Hv_true)
     del X0a
     if not math.isfinite(ai_rel) or ai_rel > minq_tol * 1.05:
         raise FloatingPointError(
@@ -697,6 +708,7 @@ def fit_reml(
     verbose: bool = True, log_detail: str = "full", return_diagnostics: bool = False,
     probe_cache: REMLProbeCache | None = None, mean_information=None,
     slq_workspace_bytes: int = 256 * 1024**2, counters: dict | None = None,
+    weighted_hv_true=None,
 ):
     """Fit single-trait Gaussian REML with AI/Fisher updates (reml.py:1254-1311).
 
@@ -808,7 +820,8 @@ def fit_reml(
         xmat = xmat.contiguous()
 
     if param_init is not None:
-        p0 = np.asarray(_as_host(param_init), np.float64).reshape(-1)
+        # reml.py:1452-1453 takes the initial parameters in float32
+        p0 = np.asarray(_as_host(param_init), np.float32).astype(np.float64).reshape(-1)
         if p0.size != G + E:
             raise ValueError(f"param_init length mismatch: expected {G + E}, got {p0.size}")
         if not np.all(np.isfinite(p0)):
@@ -879,6 +892,7 @@ def fit_reml(
         precond_conf=precond_conf, diag_atoms=diag_atoms, residual_diag_atoms=residual_diag_atoms,
         affine_slq_cache=affine, slq_reference_runtime=slq_ref,
         slq_workspace_bytes=int(slq_workspace_bytes), counters=counters,
+        weighted_hv_true=weighted_hv_true,
     )
     warm_all = torch.full_like(rhs_const, float("nan"))
     warm_ai = torch.full((n, G + E), float("nan"), device=dev)

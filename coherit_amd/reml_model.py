@@ -91,6 +91,7 @@ class _OperatorBundle:
     stacked_kv: object
     residual_diag_list: tuple | None = None
     projected_core_atoms: object | None = None
+    weighted_hv_true: object | None = None
 
 
 def _as_list(x):
@@ -156,29 +157,40 @@ class InfinitesimalREMLFitter:
             st = self._partitioned_streamer
             K_mvs = tuple((lambda V, g=g: st.component_kv(V, g, normalize=True))
                           for g in range(st.n_components))
+            rp = st.residual_precision
             return _OperatorBundle(
                 K_mvs=K_mvs, diag_list=tuple(st.component_diag_list()),
                 weighted_hv=lambda tg, te, V: st.weighted_component_hv(tg, te, V),
                 stacked_kv=lambda V: st.stacked_component_kv(V, normalize=True),
                 projected_core_atoms=lambda U: st.build_projected_core_atoms(U, subtract_identity=False),
+                weighted_hv_true=(None if rp == st.precision else
+                                  (lambda tg, te, V: st.weighted_component_hv(tg, te, V, precision=rp))),
             )
         sts = tuple(self.streamers)
         K_mvs = tuple((lambda V, st=st: st.kv(V)) for st in sts)
         diag_list = tuple(st.diag() for st in sts)
         if len(sts) == 1:
+            st = sts[0]
+            rp = st.residual_precision
             return _OperatorBundle(K_mvs=K_mvs, diag_list=diag_list, weighted_hv=None,
-                                   stacked_kv=None, projected_core_atoms=None)
+                                   stacked_kv=None, projected_core_atoms=None,
+                                   weighted_hv_true=(None if rp == st.precision else
+                                                     (lambda tg, te, V: st.weighted_component_hv(
+                                                         tg, te, V, precision=rp))))
 
-        def weighted_hv(tg, te, V):
+        def weighted_hv(tg, te, V, refine=False):
             tg = np.asarray(tg, np.float64).reshape(-1)
             acc = None
             for g, st in enumerate(sts):
                 # one unpartitioned streamer per GRM: component 0 spans all SNPs
-                term = st.weighted_component_hv([tg[g]], None, V)
+                prec = st.residual_precision if refine else None
+                term = st.weighted_component_hv([tg[g]], None, V, precision=prec)
                 acc = term if acc is None else acc + term
             if te is not None:
                 acc = acc + float(np.asarray(te).reshape(-1)[0]) * V
             return acc
+
+        refine = any(st.residual_precision != st.precision for st in sts)
 
         def stacked_kv(V):
             return torch.stack([st.kv(V) for st in sts])
@@ -187,7 +199,9 @@ class InfinitesimalREMLFitter:
             return torch.stack([st.build_projected_core_atom(U, subtract_identity=False) for st in sts])
 
         return _OperatorBundle(K_mvs=K_mvs, diag_list=diag_list, weighted_hv=weighted_hv,
-                               stacked_kv=stacked_kv, projected_core_atoms=core_atoms)
+                               stacked_kv=stacked_kv, projected_core_atoms=core_atoms,
+                               weighted_hv_true=((lambda tg, te, V: weighted_hv(tg, te, V, True))
+                                                 if refine else None))
 
     # ------------------------------------------------------ preconditioner
     @staticmethod
@@ -285,6 +299,7 @@ class InfinitesimalREMLFitter:
                 max_linesearch_trials=cfg.strict_max_linesearch_trials,
                 return_diagnostics=True, verbose=cfg.verbose,
                 probe_cache=self._reml_probe_cache, counters=counters,
+                weighted_hv_true=ops.weighted_hv_true,
             )
             vc, history, diagnostics = out
             reps.append((vc, history))

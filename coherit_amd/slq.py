@@ -19,9 +19,17 @@ EPS32 = float(np.finfo(np.float32).eps)
 
 
 def default_workspace_bytes(gpu_budget_bytes=None):
-    """slq.py:16-21."""
-    return (min(2 * 1024**3, max(1, int(gpu_budget_bytes / 8)))
-            if gpu_budget_bytes is not None else 256 * 1024**2)
+    """SLQ retained-state budget: one eighth of the GPU budget (slq.py:16-21).
+
+    The reference caps this at 2 GiB and uses 256 MiB without a budget; on a
+    288 GB device the default budget is half of the device memory and there
+    is no cap, so a whole probe block (e.g. 64 probes x 40 steps at n = 1e5)
+    runs as one batch.  Batching does not change the SLQ value or gradient.
+    """
+    if gpu_budget_bytes is None:
+        gpu_budget_bytes = 0.5 * torch.cuda.get_device_properties(
+            torch.cuda.current_device()).total_memory
+    return max(1, int(gpu_budget_bytes / 8))
 
 
 def workspace_layout(n, nsamples, m, workspace_bytes, *, itemsize=4):

@@ -196,6 +196,22 @@ int coh_lanczos_step(int64_t n, int64_t k, const float* dev_v, const float* dev_
                      double* dev_alpha, double* dev_beta, void* dev_workspace,
                      uint64_t workspace_bytes, void* stream);
 
+/* ---------------------------------------------------------------------
+ * Kernel timing instrumentation (the reference only logs wall-clock
+ * per-iteration fields, reml.py:1979-1980).  When enabled, each main
+ * contraction kernel launch is bracketed by HIP events on its stream and
+ * its algorithmic work is recorded.
+ * ------------------------------------------------------------------- */
+int coh_timing_enable(int32_t on);
+/* Drain pending events and clear the totals. */
+int coh_timing_reset(void);
+/* Totals per kernel name (names: max_kernels x 64 chars): launches, event
+ * milliseconds, algorithmic packed bytes, algorithmic FP32-equivalent
+ * flops, executed matrix-core (or VALU) operations. */
+int coh_timing_read(int32_t max_kernels, int32_t* host_n_kernels, char* host_names,
+                    int64_t* host_launches, double* host_ms, double* host_bytes,
+                    double* host_flops, double* host_ops);
+
 #ifdef __cplusplus
 }
 #endif

@@ -7,7 +7,7 @@ from __future__ import annotations
 
 import numpy as np
 
-from .geno import OraclePanel
+from .geno import OraclePanel, unpack_codes
 
 
 def _comp_slices(panel: OraclePanel):
@@ -80,6 +80,31 @@ def core_atoms(panel: OraclePanel, U, subtract_identity=True):
             core = np.zeros_like(core)
         out.append(core)
     return np.stack(out)
+
+
+def kv_packed_calls(packed_calls, widths, mean, inv_sd, V, eff, normalize=True,
+                    dtype=np.float32):
+    """Reference-structured CPU K.V over packed calls (kv_impl.py:58-113, 268-326).
+
+    For every call: unpack the (n, pw) bytes, impute missing to the mean and
+    centre (kv_impl.py:77-91), ``inner = diff^T V`` and
+    ``acc += diff @ (inv^2 * inner)`` in ``dtype`` BLAS GEMMs; finally /eff.
+    Used as the timed CPU baseline (FP32, the reference's arithmetic width).
+    """
+    V = np.asarray(V, dtype=dtype)
+    acc = np.zeros_like(V)
+    s0 = 0
+    for blk, w in zip(packed_calls, widths):
+        g = unpack_codes(blk, int(w))
+        mu = np.asarray(mean[s0 : s0 + w], dtype=dtype)[None, :]
+        iv = np.asarray(inv_sd[s0 : s0 + w], dtype=dtype)
+        diff = np.where(g == 3, mu, g.astype(dtype)) - mu
+        inner = diff.T @ V
+        acc += diff @ ((iv * iv)[:, None] * inner)
+        s0 += int(w)
+    if normalize and eff > 0:
+        acc /= dtype(eff)
+    return acc
 
 
 def exact_diag(panel: OraclePanel):

@@ -35,10 +35,18 @@ def pcg_direction_update(P, R_new, Z_new, rs_old, threshold):
     return P_new, np.where(active, rs_new, 0), rnorm, active
 
 
-def pcg_solve(Hv, B, M=None, tol=1e-2, maxiter=200, X0=None, check_every=2, counter=None):
-    """pcg.py:68-167; returns (X, max_rel_res, iters)."""
+def pcg_solve(Hv, B, M=None, tol=1e-2, maxiter=200, X0=None, check_every=2, counter=None,
+              Hv_true=None):
+    """pcg.py:68-167; returns (X, max_rel_res, iters).
+
+    ``Hv_true`` (an extension; default ``Hv``) is the operator of the true
+    residuals, which turns the verify-and-restart step into iterative
+    refinement when ``Hv`` is a cheaper, less accurate form.
+    """
     if M is None:
         M = _identity
+    if Hv_true is None:
+        Hv_true = Hv
     check_every = max(1, int(check_every))
     B = np.asarray(B)
     dt = B.dtype
@@ -56,7 +64,7 @@ def pcg_solve(Hv, B, M=None, tol=1e-2, maxiter=200, X0=None, check_every=2, coun
         return np.asarray(f(v), dtype=dt)
 
     def true_residual(value):
-        residual = ((B - op(Hv, value)) / rhs_scale).astype(dt)
+        residual = ((B - op(Hv_true, value)) / rhs_scale).astype(dt)
         norms = np.linalg.norm(residual, axis=0)
         rel = float(np.max(norms / b_norm))
         if not math.isfinite(rel):

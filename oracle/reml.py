@@ -282,7 +282,8 @@ def eval_once(pb: Problem, p, y, X, warm, *, key, tol, maxiter, slq_samples, slq
 def fit_reml(pb: Problem, y, covar, *, seed=0, h2_init=0.5, maxiter=400, minq_iter=50,
              slq_samples=30, slq_m=30, slq_mode="raw", pcg_tol=1e-3, residual_floor=1e-2,
              genetic_zero_tol=1e-8, max_linesearch_trials=3, rel_dll_tol=1e-3, precond_U=None,
-             precond_eps=1e-6, response_is_standardized=False, return_history=False):
+             precond_eps=1e-6, response_is_standardized=False, return_history=False,
+             param_init=None):
     """reml.py:1254-2092 (strict optimiser) in FP64 on dense kernels."""
     y = np.asarray(y, np.float32).reshape(-1)
     if not response_is_standardized:
@@ -299,6 +300,12 @@ def fit_reml(pb: Problem, y, covar, *, seed=0, h2_init=0.5, maxiter=400, minq_it
     tg = np.where(pb.atoms > 0, h2_init / pb.atoms.sum(), 0.0)
     te = np.full(E, (1 - h2_init) / ratoms.sum())
     p = np.concatenate([np.maximum(tg, 0), np.maximum(te, residual_floor)])
+    if param_init is not None:
+        # reml.py:1452-1470: float32 initial parameters, projected onto the bounds
+        p0 = np.asarray(param_init, np.float32).astype(np.float64).reshape(-1)
+        g0 = np.maximum(p0[:G], 0.0)
+        p = np.concatenate([np.where(g0 <= genetic_zero_tol, 0.0, g0),
+                            np.maximum(p0[G:], residual_floor)])
     _, key = probe_seeds(seed)
     affine = None
     if G == 1 and pb.res is None:

@@ -136,3 +136,72 @@ def test_errors_map_to_reference_exceptions(partitioned):
         st.kv(torch.zeros((o.n + 1, 2), device="cuda"))
     with pytest.raises(ValueError):
         st.weighted_component_hv(np.ones(2), 1.0, torch.zeros((o.n, 1), device="cuda"))
+
+
+def _col_rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return np.linalg.norm(a - b, axis=0) / np.maximum(np.linalg.norm(b, axis=0), 1e-300)
+
+
+@pytest.mark.parametrize("precision", ["i8x3", "i8x2"])
+def test_stacked_components_of_unequal_magnitude_each_meet_tolerance(precision):
+    """A rare-variant component (few SNPs, large 1/sd) beside a common one:
+    their rows of Z^T V differ ~40x in size, and each stacked output must
+    still meet the precision's tolerance column by column."""
+    rng = np.random.default_rng(5)
+    n = 900
+    p = np.concatenate([rng.uniform(0.02, 0.05, 240), rng.uniform(0.3, 0.5, 2600)])
+    codes = rng.binomial(2, p[None, :], size=(n, p.size)).astype(np.uint8)
+    codes[rng.random(codes.shape) < 0.01] = 3
+    sizes = [240, 2600]
+    st = _build(codes, precision=precision, call_width=512, component_block_sizes=sizes)
+    o = geno.OraclePanel.from_codes(codes, sizes)
+    V = rng.standard_normal((n, 20)).astype(np.float32)
+    Vd = torch.as_tensor(V, device="cuda")
+    stacked = st.stacked_component_kv(Vd).cpu().numpy()
+    tol = TOL[precision]
+    for g in range(2):
+        err = _col_rel(stacked[g], okv.component_kv(o, V, g))
+        print(precision, "component", g, "max column rel err", float(err.max()), flush=True)
+        assert err.max() < tol
+    th = np.array([0.8, 0.02])
+    err = _col_rel(st.weighted_component_hv(th, 0.3, Vd).cpu().numpy(),
+                   okv.weighted_hv(o, th, 0.3, V))
+    assert err.max() < tol
+    st.close()
+
+
+def test_fp32_operator_is_linear_on_the_intercept_at_full_scale():
+    """n=100k, m=500k.  K.1 sums ~1e5 centred codes per SNP whose total is
+    ~1e-3, so FP32 running sums lose it entirely; PCG then stalls because
+    that rounding depends on the input (the operator stops being linear).
+    The blocked FP64 accumulation keeps H.V linear on [1, r, 1 + r] to below
+    the 1e-5 that multi-component PCG is held to (measured 4.7e-6; FP32
+    running sums gave ~1e-4), and random columns within 1e-5 of the 3-limb
+    path.  What remains is the FP32 storage of Z^T V between the phases,
+    as in the reference.  (Centring in FP32 as the reference does,
+    fl(code - mean), is a fixed perturbation of Z: it does not break
+    linearity.)"""
+    from coherit_amd import synth
+    from coherit_amd.geno_stream import GenoBlockStreamer
+
+    wl = synth.workload(1)
+    st = GenoBlockStreamer(synth.SyntheticSource(wl.n, wl.p, wl.seed),
+                           component_variant_indices=wl.components)
+    r = torch.randn(wl.n, device="cuda")
+    V = torch.stack([torch.ones_like(r), r, 1.0 + r], dim=1).contiguous()
+    print("fp32 linearity n", wl.n, "m", wl.m, flush=True)
+    th = np.array([0.0186, 0.044])
+    st.set_precision("i8x3")
+    ref = st.weighted_component_hv(th, 0.064, V).double().cpu().numpy()
+    st.set_precision("fp32")
+    got = st.weighted_component_hv(th, 0.064, V).double().cpu().numpy()
+    defect = np.linalg.norm(got[:, 2] - got[:, 0] - got[:, 1]) / np.linalg.norm(got[:, 0])
+    err_r = _col_rel(got[:, 1:2], ref[:, 1:2])[0]
+    print("H.V linearity defect", defect, "random column rel err", err_r, flush=True)
+    assert defect < 1e-5
+    assert err_r < 1e-5
+    st.close()
+    del st
+    torch.cuda.empty_cache()

@@ -190,3 +190,36 @@ def test_zero_iteration_budget_returns_verified_initial_residual(be):
 def test_invalid_operator_cannot_be_reported_as_converged(be, factor):
     with pytest.raises(FloatingPointError):
         be.solve(lambda v: factor * v, be.arr(np.ones((8, 1))), maxiter=4)
+
+
+def _quantized(be, h, bits):
+    """H.V with V rounded per column to ``bits`` bits of its max magnitude: a
+    deterministic, slightly nonlinear operator like the limb contraction."""
+    h_np = np.asarray(h, np.float64)
+
+    def apply(v):
+        vv = np.asarray(be.np(v), np.float64)
+        vv = vv[:, None] if vv.ndim == 1 else vv
+        step = np.max(np.abs(vv), axis=0, keepdims=True) / 2.0**bits
+        step = np.where(step > 0, step, 1.0)
+        q = np.round(vv / step) * step
+        return be.arr(h_np @ q)
+
+    return apply
+
+
+def test_refinement_reaches_tolerance_below_operator_floor(be):
+    rng = np.random.default_rng(3)
+    n = 96
+    A = rng.standard_normal((n, n))
+    h = A @ A.T / n + np.eye(n)
+    b = be.arr(rng.standard_normal((n, 3)))
+    exact = be.matmul(be.arr(h))
+    coarse = _quantized(be, h, 10)
+    # the coarse operator alone cannot verify 1e-6
+    _, rel_c, _ = be.solve(coarse, b, tol=1e-6, maxiter=300)
+    assert float(rel_c) > 1e-6
+    x, rel, it = be.solve(coarse, b, tol=1e-6, maxiter=300, Hv_true=exact)
+    assert float(rel) <= 1e-6
+    assert float(rel) == pytest.approx(_true_relative(be, exact, x, b), rel=1e-3, abs=1e-9)
+    np.testing.assert_allclose(be.np(x), np.linalg.solve(h, be.np(b)), rtol=1e-4, atol=1e-5)
