@@ -44,6 +44,8 @@ def parse():
     ap.add_argument("--cpu-slice-snps", type=int, default=8192)
     ap.add_argument("--kv-only", type=int, default=0,
                     help="skip the fit; time this many K.V launches at PCG/SLQ widths (profiling)")
+    ap.add_argument("--calibrate-copy", type=float, default=0.0,
+                    help="only copy a buffer of this many GiB 5 times (HBM counter calibration)")
     ap.add_argument("--quiet", action="store_true")
     ap.add_argument("--verbose", action="store_true", help="log every REML iteration to stderr")
     return ap.parse_args()
@@ -191,6 +193,19 @@ def main():
 
         logging.basicConfig(level=logging.INFO, stream=sys.stderr, format="[reml] %(message)s")
     torch.cuda.set_device(0)
+    if args.calibrate_copy > 0:
+        nbytes = int(args.calibrate_copy * 2**30) // 16 * 16
+        src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / 5
+        print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes,
+                          "ms": dt * 1e3, "GBps_read_plus_write": 2 * nbytes / dt / 1e9}))
+        return
     wl, st, y, X, t_build = build(args)
     log(args, f"panel n={st.n} m={st.m} built in {t_build:.2f}s ({st.panel.nbytes() / 1e9:.1f} GB, two layouts)")
     if args.kv_only:

@@ -30,6 +30,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 
 namespace coh {
 
@@ -608,15 +609,36 @@ static unsigned grid1d(int64_t total) {
   return (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(total, 256), 8192));
 }
 
+// Launch shape per phase: NB 16-column blocks per column group (grid z),
+// MB 16-row tiles per wave, T K-steps per double-buffered LDS stage.
 struct MfmaShape {
-  int NB, MB1, MB2;
+  int NB1, MB1, T1;
+  int NB2, MB2, T2;
 };
 
-static MfmaShape shape_for(int64_t k) {
-  if (k <= 16) return {1, 4, 8};
-  if (k <= 32) return {2, 2, 4};
-  return {4, 1, 2};
+// Defaults from a sweep of every compiled shape at n = 1e5, m = 5e5 on
+// MI355X (k = 12 and 64): four-K-step LDS stages and 2-4 row tiles per wave
+// win; the phase-2 LDS budget limits three limbs to NB * T <= 5.
+static MfmaShape default_shape(int64_t k, int L) {
+  if (k <= 16) return {1, 4, 4, 1, 4, 4};
+  if (k <= 32) return {2, 2, 4, 2, 4, 2};
+  if (L == 2) return {2, 2, 4, 4, 4, 2};
+  return {2, 2, 4, 2, 4, 2};
 }
+
+// The default table, or COH_MFMA_SHAPE="nb1,mb1,t1,nb2,mb2,t2" for tuning
+// sweeps (a shape with no compiled kernel is rejected when it launches).
+static MfmaShape shape_for(int64_t k, int L) {
+  MfmaShape s = default_shape(k, L);
+  if (const char* e = std::getenv("COH_MFMA_SHAPE")) {
+    MfmaShape o;
+    if (std::sscanf(e, "%d,%d,%d,%d,%d,%d", &o.NB1, &o.MB1, &o.T1, &o.NB2, &o.MB2, &o.T2) == 6)
+      s = o;
+  }
+  return s;
+}
+
+static int64_t col_blocks(int64_t k, int NB) { return ceil_div(ceil_div(k, 16), NB) * NB; }
 
 constexpr int kColmaxBlocks = 256;
 
@@ -625,10 +647,20 @@ int plan_mfma(const coh_panel* p, int precision, int64_t k, int32_t cb, int32_t 
   pl->slot0 = p->comp_slot_begin[cb];
   pl->slot1 = p->comp_slot_end[ce - 1];
   const int64_t nslots = pl->slot1 - pl->slot0;
-  const MfmaShape sh = shape_for(k);
-  pl->KC = sh.NB * 16;
-  const int64_t groups = ceil_div(ceil_div(k, 16), sh.NB);
-  const int64_t NBt = groups * sh.NB;  // column blocks padded to whole groups (zero-filled)
+  const MfmaShape sh = shape_for(k, pl->limbs);
+  if ((sh.MB1 != 1 && sh.MB1 != 2 && sh.MB1 != 4) || sh.NB1 < 1 || sh.NB2 < 1 || sh.MB2 < 1 ||
+      sh.T1 < 1 || sh.T2 < 1 || 4 % sh.T1 != 0 || 4 % sh.T2 != 0)
+    return fail(COH_ERR_VALUE, "invalid MFMA launch shape");
+  pl->shape[0] = sh.NB1;
+  pl->shape[1] = sh.MB1;
+  pl->shape[2] = sh.T1;
+  pl->shape[3] = sh.NB2;
+  pl->shape[4] = sh.MB2;
+  pl->shape[5] = sh.T2;
+  pl->KC = sh.NB1 * 16;
+  // column blocks padded to whole groups (zero-filled)
+  const int64_t NBt1 = col_blocks(k, sh.NB1), NBt2 = col_blocks(k, sh.NB2);
+  const int64_t groups = NBt1 / sh.NB1, groups2 = NBt2 / sh.NB2;
   const int64_t NSB = ceil_div(p->n, 256);  // 256-sample super-blocks
   const int64_t KB1 = NSB * 4;
   pl->n_pad = NSB * 256;
@@ -640,7 +672,7 @@ int plan_mfma(const coh_panel* p, int precision, int64_t k, int32_t cb, int32_t 
   pl->rows_per_split = ceil_div(NSB, nsplit);  // in super-blocks
   pl->nsplit1 = static_cast<int>(ceil_div(NSB, pl->rows_per_split));
   // phase-2 slot chunks
-  const int64_t xb2 = std::max<int64_t>(1, ceil_div(p->n, 64 * sh.MB2)) * groups;
+  const int64_t xb2 = std::max<int64_t>(1, ceil_div(p->n, 64 * sh.MB2)) * groups2;
   const int64_t target_chunks = std::max<int64_t>(1, ceil_div(kTargetBlocks, xb2));
   pl->chunk_len = std::max<int64_t>(256, round_up(ceil_div(nslots, target_chunks), 256));
   build_chunks(p, cb, ce, pl->chunk_len, &pl->tab);
@@ -650,9 +682,9 @@ int plan_mfma(const coh_panel* p, int precision, int64_t k, int32_t cb, int32_t 
   pl->off_U = off;
   off += round_up(nslots * k * 4, 256);
   pl->off_q = off;  // Bq (phase 1) then Bu (phase 2) share nothing: both kept
-  off += round_up((int64_t)L * NBt * KB1 * 64 * 16, 256);
+  off += round_up((int64_t)L * NBt1 * KB1 * 64 * 16, 256);
   pl->off_p2 = off;  // Bu
-  off += round_up((int64_t)L * NBt * (nslots / 64) * 2 * 64 * 16, 256);
+  off += round_up((int64_t)L * NBt2 * (nslots / 64) * 2 * 64 * 16, 256);
   pl->off_aux = off;  // scales and inverse scales per (component, column), colmax partials
   off += round_up(3 * (int64_t)std::max(1, pl->tab.ncomp) * k * 8, 256) +
          round_up((int64_t)kColmaxBlocks * k * 4, 256) +
@@ -710,9 +742,9 @@ static int phase1(const coh_panel* p, const KvPlan& pl, const float* V, int64_t 
                   const CompScale& cs, int inv_pow, uint8_t* ws, hipStream_t st) {
   const double qmax = (L == 3) ? kQ3 : kQ2;
   const int64_t nslots = pl.slot1 - pl.slot0;
-  const MfmaShape sh = shape_for(k);
-  const int64_t groups = ceil_div(ceil_div(k, 16), sh.NB);
-  const int64_t NBt = groups * sh.NB;
+  const int NB = pl.shape[0], MB = pl.shape[1], T = pl.shape[2];
+  const int64_t NBt = col_blocks(k, NB);
+  const int64_t groups = NBt / NB;
   const int64_t KB = pl.n_pad / 64;
   const ScaleView sv = scale_view(pl, k, ws);
   int s = colscale(V, p->n, k, ldv, nullptr, 0, qmax, sv.scale, sv.inv_scale, nullptr, sv.part,
@@ -742,18 +774,25 @@ static int phase1(const coh_panel* p, const KvPlan& pl, const float* V, int64_t 
   a.U = reinterpret_cast<float*>(ws + pl.off_U);
   a.part = reinterpret_cast<int32_t*>(ws + pl.off_p1);
   a.nsplit = pl.nsplit1;
-  dim3 grid((unsigned)(nslots / (64 * sh.MB1)), (unsigned)pl.nsplit1, (unsigned)groups);
+  dim3 grid((unsigned)(nslots / (64 * MB)), (unsigned)pl.nsplit1, (unsigned)groups);
   char name[64];
-  snprintf(name, sizeof(name), "k_p1_mfma_L%d_NB%d", L, sh.NB);
+  snprintf(name, sizeof(name), "k_p1_mfma_L%d_NB%d_MB%d_T%d", L, NB, MB, T);
   KernelTimer tm(st, name, (double)p->n * (double)ceil_div(pl.m_sel, 4),
                  2.0 * (double)p->n * (double)pl.m_sel * (double)k,
                  2.0 * L * 2.0 * (double)nslots * (double)pl.n_pad * (double)(NBt * 16));
-  if (sh.NB == 1)
-    hipLaunchKernelGGL((k_p1_mfma<L, 1, 4, 4>), grid, dim3(256), 0, st, a);
-  else if (sh.NB == 2)
-    hipLaunchKernelGGL((k_p1_mfma<L, 2, 2, 1>), grid, dim3(256), 0, st, a);
-  else
-    hipLaunchKernelGGL((k_p1_mfma<L, 4, 1, 1>), grid, dim3(256), 0, st, a);
+  bool launched = false;
+#define COH_P1V(NB_, MB_, T_)                                                      \
+  if constexpr (L * NB_ * T_ <= 32)                                                \
+    if (!launched && NB == NB_ && MB == MB_ && T == T_) {                          \
+      hipLaunchKernelGGL((k_p1_mfma<L, NB_, MB_, T_>), grid, dim3(256), 0, st, a); \
+      launched = true;                                                             \
+    }
+  COH_P1V(1, 4, 4) COH_P1V(1, 2, 4) COH_P1V(1, 4, 2)
+  COH_P1V(2, 2, 1) COH_P1V(2, 2, 2) COH_P1V(2, 2, 4) COH_P1V(2, 4, 1) COH_P1V(2, 4, 2)
+  COH_P1V(2, 1, 4) COH_P1V(4, 1, 1) COH_P1V(4, 1, 2) COH_P1V(4, 1, 4) COH_P1V(4, 2, 1)
+  COH_P1V(4, 2, 2) COH_P1V(4, 2, 4)
+#undef COH_P1V
+  if (!launched) return fail(COH_ERR_VALUE, "no compiled phase-1 kernel for this MFMA shape");
   tm.stop();
   COH_CHECK_LAUNCH();
   if (pl.nsplit1 > 1) {
@@ -768,9 +807,9 @@ static int phase2(const coh_panel* p, const KvPlan& pl, int64_t k, int stacked, 
                   int64_t ldo, int64_t out_stride, float beta, uint8_t* ws, hipStream_t st) {
   const double qmax = (L == 3) ? kQ3 : kQ2;
   const int64_t nslots = pl.slot1 - pl.slot0;
-  const MfmaShape sh = shape_for(k);
-  const int64_t groups = ceil_div(ceil_div(k, 16), sh.NB);
-  const int64_t NBt = groups * sh.NB;
+  const int NB = pl.shape[3], MB = pl.shape[4], T = pl.shape[5];
+  const int64_t NBt = col_blocks(k, NB);
+  const int64_t groups = NBt / NB;
   const int64_t KB = nslots / 64;
   const float* U = reinterpret_cast<const float*>(ws + pl.off_U);
   // one fixed-point scale per (component, column): the rows of U carry each
@@ -803,19 +842,25 @@ static int phase2(const coh_panel* p, const KvPlan& pl, int64_t k, int stacked, 
   a.scale = scale;
   a.tab = pl.tab;
   a.part = reinterpret_cast<int32_t*>(ws + pl.off_p1);
-  dim3 grid((unsigned)ceil_div(p->n, 64 * sh.MB2), (unsigned)pl.tab.nchunks, (unsigned)groups);
+  dim3 grid((unsigned)ceil_div(p->n, 64 * MB), (unsigned)pl.tab.nchunks, (unsigned)groups);
   char name[64];
-  snprintf(name, sizeof(name), "k_p2_mfma_L%d_NB%d", L, sh.NB);
+  snprintf(name, sizeof(name), "k_p2_mfma_L%d_NB%d_MB%d_T%d", L, NB, MB, T);
   KernelTimer tm(st, name, (double)p->n * (double)ceil_div(pl.m_sel, 4),
                  2.0 * (double)p->n * (double)pl.m_sel * (double)k,
-                 2.0 * L * 2.0 * (double)nslots * (double)round_up(p->n, 64 * sh.MB2) *
+                 2.0 * L * 2.0 * (double)nslots * (double)round_up(p->n, 64 * MB) *
                      (double)(NBt * 16));
-  if (sh.NB == 1)
-    hipLaunchKernelGGL((k_p2_mfma<L, 1, 8, 4>), grid, dim3(256), 0, st, a);
-  else if (sh.NB == 2)
-    hipLaunchKernelGGL((k_p2_mfma<L, 2, 4, 1>), grid, dim3(256), 0, st, a);
-  else
-    hipLaunchKernelGGL((k_p2_mfma<L, 4, 2, 1>), grid, dim3(256), 0, st, a);
+  bool launched = false;
+#define COH_P2V(NB_, MB_, T_)                                                      \
+  if constexpr (L * NB_ * T_ <= 16)                                                \
+    if (!launched && NB == NB_ && MB == MB_ && T == T_) {                          \
+      hipLaunchKernelGGL((k_p2_mfma<L, NB_, MB_, T_>), grid, dim3(256), 0, st, a); \
+      launched = true;                                                             \
+    }
+  COH_P2V(1, 8, 4) COH_P2V(1, 4, 4) COH_P2V(1, 8, 2) COH_P2V(2, 4, 1) COH_P2V(2, 4, 2)
+  COH_P2V(2, 8, 1) COH_P2V(2, 8, 2) COH_P2V(4, 2, 1) COH_P2V(4, 2, 2) COH_P2V(4, 4, 1)
+  COH_P2V(4, 4, 2) COH_P2V(4, 1, 2)
+#undef COH_P2V
+  if (!launched) return fail(COH_ERR_VALUE, "no compiled phase-2 kernel for this MFMA shape");
   tm.stop();
   COH_CHECK_LAUNCH();
   const int nout = stacked ? pl.tab.ncomp : 1;

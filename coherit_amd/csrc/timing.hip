@@ -72,6 +72,12 @@ KernelTimer::KernelTimer(hipStream_t st, const char* name, double bytes, double 
   active_ = true;
 }
 
+KernelTimer::~KernelTimer() {
+  if (!active_) return;
+  hipEventDestroy(start_);
+  hipEventDestroy(stop_);
+}
+
 void KernelTimer::stop() {
   if (!active_) return;
   hipEventRecord(stop_, st_);

@@ -10,6 +10,9 @@ namespace coh {
 class KernelTimer {
  public:
   KernelTimer(hipStream_t st, const char* name, double bytes, double flops, double ops);
+  ~KernelTimer();  // drops a bracket that was never stopped (a launch that failed)
+  KernelTimer(const KernelTimer&) = delete;
+  KernelTimer& operator=(const KernelTimer&) = delete;
   void stop();
 
  private:

@@ -205,3 +205,40 @@ def test_fp32_operator_is_linear_on_the_intercept_at_full_scale():
     st.close()
     del st
     torch.cuda.empty_cache()
+
+
+P1_SHAPES = ["1,4,4", "1,2,4", "1,4,2", "2,2,1", "2,2,2", "2,2,4", "2,4,1", "2,4,2", "2,1,4",
+             "4,1,1", "4,1,2", "4,1,4", "4,2,1", "4,2,2", "4,2,4"]
+P2_SHAPES = ["1,8,4", "1,4,4", "1,8,2", "2,4,1", "2,4,2", "2,8,1", "2,8,2", "4,2,1", "4,2,2",
+             "4,4,1", "4,4,2", "4,1,2"]
+
+
+def _p1_fits(L, s):
+    nb, _, t = map(int, s.split(","))
+    return L * nb * t <= 32
+
+
+def _p2_fits(L, s):
+    nb, _, t = map(int, s.split(","))
+    return L * nb * t <= 16
+
+
+@pytest.mark.parametrize("precision,L", [("i8x2", 2), ("i8x3", 3)])
+def test_every_compiled_mfma_shape_gives_identical_results(partitioned, monkeypatch, precision, L):
+    """Launch shapes only change tiling: integer contractions and fixed-order
+    reductions make every compiled shape bit-identical to the default."""
+    st, o = partitioned
+    st.set_precision(precision)
+    rng = np.random.default_rng(9)
+    V = torch.as_tensor(rng.standard_normal((o.n, 37)).astype(np.float32), device="cuda")
+    ref = st.stacked_component_kv(V).cpu().numpy()
+    p1 = [s for s in P1_SHAPES if _p1_fits(L, s)]
+    p2 = [s for s in P2_SHAPES if _p2_fits(L, s)]
+    for i in range(max(len(p1), len(p2))):
+        shape = f"{p1[i % len(p1)]},{p2[i % len(p2)]}"
+        monkeypatch.setenv("COH_MFMA_SHAPE", shape)
+        print(precision, "shape", shape, flush=True)
+        got = st.stacked_component_kv(V).cpu().numpy()
+        np.testing.assert_array_equal(got, ref)
+    monkeypatch.delenv("COH_MFMA_SHAPE")
+    st.set_precision("fp32")
