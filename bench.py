@@ -73,6 +73,19 @@ def build(args):
     return wl, st, y, X, t_build
 
 
+TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic.json")
+
+
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of ``kernel`` from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (copy-calibrated), or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            return json.load(f)["bytes_per_launch"].get(kernel)
+    except (OSError, ValueError, KeyError):
+        return None
+
+
 def roofline(timing: dict):
     """Dominant kernel (largest event time) and its roofline numbers."""
     if not timing:
@@ -96,7 +109,7 @@ def roofline(timing: dict):
         roof = {"bound": "mfma", "achieved": round(ach, 2), "peak": peak_c, "unit": "TFLOP/s",
                 "frac": round(ach / peak_c, 4)}
     roof.update({
-        "traffic": None, "kernel": name, "launches": t["launches"],
+        "traffic": measured_traffic(name), "kernel": name, "launches": t["launches"],
         "avg_launch_ms": round(per_launch * 1e3, 4), "alg_bytes_per_launch": bytes_l,
         "alg_flops_per_launch": flops_l,
         "matrix_core_util": round(ops_l / per_launch / 1e12 / peak_c, 4) if mfma else None,

@@ -149,3 +149,33 @@ def test_split_form_fit_refines_pcg_and_stays_within_tolerance():
     assert abs(res.heritability - _h2(th_o, atoms)) <= 1e-3
     np.testing.assert_allclose(res.var_components, th_o, rtol=1e-2, atol=1e-4)
     fitter.close()
+
+
+def test_slq_value_and_gradient_do_not_depend_on_probe_batching():
+    """workspace_bytes only sets how many probes share a batch (slq.py:24-41;
+    tests/test_multi_slq_score_consistency.py:100-107)."""
+    from coherit_amd import slq
+
+    rng = np.random.default_rng(12)
+    n = 200
+    Ks = []
+    for w in (30, 55):
+        Z = rng.standard_normal((n, w))
+        Ks.append(torch.as_tensor(Z @ Z.T / w, dtype=torch.float32, device="cuda"))
+    th = [0.4, 0.3, 0.5]
+    H = th[0] * Ks[0] + th[1] * Ks[1] + th[2] * torch.eye(n, device="cuda")
+
+    def mv(V):
+        return H @ V
+
+    def pull(left, right):
+        prods = [Ks[0] @ left, Ks[1] @ left, left]
+        return H @ left, torch.stack([(right * p).sum() for p in prods]).double().cpu().numpy()
+
+    out = []
+    for ws in (10**12, slq.workspace_layout(n, 16, 12, 1)["peak_bytes"] // 2 + 1, 1):
+        print("workspace", ws, slq.workspace_layout(n, 16, 12, ws)["batch_width"], flush=True)
+        out.append(slq.logdet_value_and_grad(mv, pull, n, 5, nsamples=16, m=12, workspace_bytes=ws))
+    for v, g in out[1:]:
+        assert float(v) == pytest.approx(float(out[0][0]), rel=1e-5)
+        np.testing.assert_allclose(np.asarray(g), np.asarray(out[0][1]), rtol=1e-4)

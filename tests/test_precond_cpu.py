@@ -1,0 +1,98 @@
+"""Projected-core preconditioner M = d I + U C U^T (precond.py:42-221).
+
+The oracle (oracle/reml.py ProjectedCore) is pinned against dense linear
+algebra: M^{-1}(M v) = v, (M^{-1/2})^2 = M^{-1}, log|M| = slogdet(M).  The
+product's device-agnostic torch code then runs on CPU tensors against the
+oracle (the device path runs the same torch ops on the GPU in the REML
+tests)."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import reml as oreml
+
+
+def _problem(n=60, r=9, G=2, seed=4):
+    rng = np.random.default_rng(seed)
+    U, _ = np.linalg.qr(rng.standard_normal((n, r)))
+    Ks, atoms = [], []
+    for g in range(G):
+        Z = rng.standard_normal((n, 40 + 10 * g))
+        K = Z @ Z.T / Z.shape[1]
+        Ks.append(K)
+        atoms.append(float(np.trace(K) / n))
+    return U, Ks, np.asarray(atoms)
+
+
+def _dense_M(core, tg, te, eps):
+    d, mid = core.runtime(tg, te, eps)
+    U = core.U
+    return d * (np.eye(U.shape[0]) - U @ U.T) + U @ mid @ U.T, d
+
+
+@pytest.mark.parametrize("tg", [[0.3, 0.2], [0.0, 0.7], [1.5, 0.01]])
+def test_oracle_projected_core_closed_forms(tg):
+    U, Ks, atoms = _problem()
+    core = oreml.ProjectedCore(U, Ks, atoms)
+    tg = np.asarray(tg)
+    te = np.array([0.4])
+    M, d = _dense_M(core, tg, te, 1e-6)
+    v = np.random.default_rng(1).standard_normal((U.shape[0], 3))
+    np.testing.assert_allclose(core.apply_inv(tg, te, 1e-6)(M @ v), v, rtol=1e-10, atol=1e-10)
+    W, logdet = core.invsqrt(tg, te, 1e-6)
+    np.testing.assert_allclose(W(W(M @ v)), v, rtol=1e-9, atol=1e-9)
+    sign, ref = np.linalg.slogdet(M)
+    assert sign > 0
+    assert logdet == pytest.approx(ref, rel=1e-12, abs=1e-9)
+
+
+@pytest.mark.parametrize("tg", [[0.3, 0.2], [0.0, 0.7]])
+def test_product_preconditioner_matches_oracle_on_cpu(tg):
+    from coherit_amd import precond
+
+    U, Ks, atoms = _problem()
+    core = oreml.ProjectedCore(U, Ks, atoms)
+    tg = np.asarray(tg)
+    te = 0.4
+    conf = precond.ProjectedCorePrecondConf(
+        U=torch.as_tensor(U, dtype=torch.float32),
+        core_atoms=torch.as_tensor(np.stack(core.cores)), total_rank=U.shape[1], n_grm=2,
+        diag_mode="scalar_identity", diag_atoms=torch.as_tensor(atoms))
+    diag_H = precond.scalar_diag_from_precond_conf(conf, tg, [te])
+    assert diag_H == pytest.approx(float(tg @ atoms + te))
+    rt = precond.build_projected_core_runtime(conf, tg, diag_H, eps=1e-6, need_invsqrt=True)
+    v = np.random.default_rng(2).standard_normal((U.shape[0], 4))
+    vt = torch.as_tensor(v, dtype=torch.float32)
+    got = precond.make_projected_core_precond_from_runtime(rt)(vt).double().numpy()
+    ref = core.apply_inv(tg, np.array([te]), 1e-6)(v)
+    np.testing.assert_allclose(got, ref, rtol=2e-5, atol=2e-6)
+    W, logdet = core.invsqrt(tg, np.array([te]), 1e-6)
+    got_w = precond.projected_core_apply_invsqrt(rt, vt).double().numpy()
+    np.testing.assert_allclose(got_w, W(v), rtol=2e-5, atol=2e-6)
+    assert precond.projected_core_logdet(rt, U.shape[0]) == pytest.approx(logdet, rel=1e-10)
+
+
+def test_non_positive_core_raises_floating_point_error():
+    from coherit_amd import precond
+
+    U, Ks, atoms = _problem()
+    r = U.shape[1]
+    conf = precond.ProjectedCorePrecondConf(
+        U=torch.as_tensor(U, dtype=torch.float32),
+        core_atoms=torch.as_tensor(np.stack([-5 * np.eye(r), np.zeros((r, r))])), total_rank=r,
+        n_grm=2, diag_mode="scalar_identity")
+    with pytest.raises(FloatingPointError):
+        precond.build_projected_core_runtime(conf, [1.0, 0.0], 1.0)
+
+
+def test_cholesky_qr2_basis_is_orthonormal_with_the_same_range():
+    from coherit_amd.precond import orthonormal_basis
+
+    rng = np.random.default_rng(3)
+    Y = torch.as_tensor(rng.standard_normal((500, 40)) @ np.diag(np.linspace(1, 30, 40)),
+                        dtype=torch.float32)
+    Q = orthonormal_basis(Y).double()
+    np.testing.assert_allclose((Q.T @ Q).numpy(), np.eye(40), atol=2e-6)
+    Yd = Y.double()
+    resid = Yd - Q @ (Q.T @ Yd)
+    assert float(resid.norm() / Yd.norm()) < 1e-6
