@@ -265,10 +265,12 @@ struct P1Args {
   int nsplit;
 };
 
-template <int L, int NB, int MB, int T>
+template <int L, int NB, int MB, int T, int PD>
 __global__ __launch_bounds__(256) void k_p1_mfma(P1Args a) {
   // Double-buffered LDS stages of T K-steps of the quantised B block, shared
   // by the 4 waves; the next stage's global loads overlap this stage's MFMAs.
+  // The genotype rows (A) stream from HBM with no reuse: each wave keeps PD
+  // super-blocks of them in flight in a register pipe.
   constexpr int STG = L * NB * T * 64;
   constexpr int PER = (STG + 255) / 256;
   __shared__ v4i sB[2][STG];
@@ -316,26 +318,40 @@ __global__ __launch_bounds__(256) void k_p1_mfma(P1Args a) {
       if (idx < STG) sB[buf][idx] = reg[p];
     }
   };
-  v4i cur[MB], nxt[MB];
+  v4i pipe[PD + 1][MB];  // pipe[d] = super-block sb + d
+#pragma unroll
+  for (int d = 0; d <= PD; ++d)
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb) pipe[d][mb] = v4i{0, 0, 0, 0};
+  // Loads are unconditional, with indices clamped into this split, so the
+  // compiler can count them: the wait before each LDS store of a B stage (an
+  // in-order vmcnt) then leaves the later genotype prefetch in flight.  A
+  // clamped genotype load beyond sb1 fills a pipe slot that is never consumed;
+  // a clamped extra B stage is stored into the idle LDS buffer.
   if (sb0 < sb1) {
 #pragma unroll
-    for (int mb = 0; mb < MB; ++mb) cur[mb] = *reinterpret_cast<const v4i*>(rowp[mb] + sb0 * 64);
+    for (int d = 0; d < PD; ++d) {
+      const int64_t sbl = min(sb0 + d, sb1 - 1);
+#pragma unroll
+      for (int mb = 0; mb < MB; ++mb)
+        pipe[d][mb] = *reinterpret_cast<const v4i*>(rowp[mb] + sbl * 64);
+    }
     gload(sb0 * 4);
     sstore(0);
   }
   __syncthreads();
   int buf = 0;
   for (int64_t sb = sb0; sb < sb1; ++sb) {
-    if (sb + 1 < sb1) {
-#pragma unroll
-      for (int mb = 0; mb < MB; ++mb)
-        nxt[mb] = *reinterpret_cast<const v4i*>(rowp[mb] + (sb + 1) * 64);
-    }
 #pragma unroll
     for (int st = 0; st < 4 / T; ++st) {
       const int64_t ks = sb * 4 + st * T;
-      const bool more = ks + T < ks_end;
-      if (more) gload(ks + T);
+      gload(min(ks + T, ks_end - T));
+      if (st == 0) {
+        const int64_t sbl = min(sb + PD, sb1 - 1);
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb)
+          pipe[PD][mb] = *reinterpret_cast<const v4i*>(rowp[mb] + sbl * 64);
+      }
 #pragma unroll
       for (int tt = 0; tt < T; ++tt) {
         const int t = st * T + tt;
@@ -347,7 +363,7 @@ __global__ __launch_bounds__(256) void k_p1_mfma(P1Args a) {
 #pragma unroll
         for (int mb = 0; mb < MB; ++mb) {
           v4i ag, ao;
-          decode16(static_cast<uint32_t>(cur[mb][t]), ag, ao);
+          decode16(static_cast<uint32_t>(pipe[0][mb][t]), ag, ao);
 #pragma unroll
           for (int l = 0; l < L; ++l)
 #pragma unroll
@@ -359,12 +375,14 @@ __global__ __launch_bounds__(256) void k_p1_mfma(P1Args a) {
             }
         }
       }
-      if (more) sstore(buf ^ 1);
+      sstore(buf ^ 1);
       __syncthreads();
       buf ^= 1;
     }
 #pragma unroll
-    for (int mb = 0; mb < MB; ++mb) cur[mb] = nxt[mb];
+    for (int d = 0; d < PD; ++d)
+#pragma unroll
+      for (int mb = 0; mb < MB; ++mb) pipe[d][mb] = pipe[d + 1][mb];
   }
   // C layout: lane holds column (lane % 16), rows 4*(lane/16) + v.
 #pragma unroll
@@ -446,7 +464,7 @@ struct P2Args {
   int32_t* part;       // [chunk][n][k][L]
 };
 
-template <int L, int NB, int MB, int T>
+template <int L, int NB, int MB, int T, int PD>
 __global__ __launch_bounds__(256) void k_p2_mfma(P2Args a) {
   // Double-buffered LDS stages of T K-steps (u and mean*u halves).
   constexpr int STG = L * NB * T * 2 * 64;
@@ -495,30 +513,37 @@ __global__ __launch_bounds__(256) void k_p2_mfma(P2Args a) {
       if (idx < STG) sB[buf][idx] = reg[p];
     }
   };
-  v4i cur[MB], nxt[MB];
+  v4i pipe[PD + 1][MB];  // pipe[d] = super-block sb + d (zero for rows >= n)
 #pragma unroll
-  for (int mb = 0; mb < MB; ++mb) cur[mb] = v4i{0, 0, 0, 0};
+  for (int d = 0; d <= PD; ++d)
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb) pipe[d][mb] = v4i{0, 0, 0, 0};
+  // Unconditional clamped loads as in phase 1; rows >= n read row 0 (their
+  // results are never stored).
   if (sb0 < sb1) {
 #pragma unroll
-    for (int mb = 0; mb < MB; ++mb)
-      if (rowok[mb]) cur[mb] = *reinterpret_cast<const v4i*>(rowp[mb] + sb0 * 64);
+    for (int d = 0; d < PD; ++d) {
+      const int64_t sbl = min(sb0 + d, sb1 - 1);
+#pragma unroll
+      for (int mb = 0; mb < MB; ++mb)
+        pipe[d][mb] = *reinterpret_cast<const v4i*>(rowp[mb] + sbl * 64);
+    }
     gload(sb0 * 4);
     sstore(0);
   }
   __syncthreads();
   int buf = 0;
   for (int64_t sb = sb0; sb < sb1; ++sb) {
-    if (sb + 1 < sb1) {
-#pragma unroll
-      for (int mb = 0; mb < MB; ++mb)
-        nxt[mb] = rowok[mb] ? *reinterpret_cast<const v4i*>(rowp[mb] + (sb + 1) * 64)
-                            : v4i{0, 0, 0, 0};
-    }
 #pragma unroll
     for (int st = 0; st < 4 / T; ++st) {
       const int64_t ks = sb * 4 + st * T;
-      const bool more = ks + T < ks_end;
-      if (more) gload(ks + T);
+      gload(min(ks + T, ks_end - T));
+      if (st == 0) {
+        const int64_t sbl = min(sb + PD, sb1 - 1);
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb)
+          pipe[PD][mb] = *reinterpret_cast<const v4i*>(rowp[mb] + sbl * 64);
+      }
 #pragma unroll
       for (int tt = 0; tt < T; ++tt) {
         const int t = st * T + tt;
@@ -534,7 +559,7 @@ __global__ __launch_bounds__(256) void k_p2_mfma(P2Args a) {
 #pragma unroll
         for (int mb = 0; mb < MB; ++mb) {
           v4i ag, am;
-          decode16m(static_cast<uint32_t>(cur[mb][t]), ag, am);
+          decode16m(static_cast<uint32_t>(pipe[0][mb][t]), ag, am);
 #pragma unroll
           for (int l = 0; l < L; ++l)
 #pragma unroll
@@ -546,12 +571,14 @@ __global__ __launch_bounds__(256) void k_p2_mfma(P2Args a) {
             }
         }
       }
-      if (more) sstore(buf ^ 1);
+      sstore(buf ^ 1);
       __syncthreads();
       buf ^= 1;
     }
 #pragma unroll
-    for (int mb = 0; mb < MB; ++mb) cur[mb] = nxt[mb];
+    for (int d = 0; d < PD; ++d)
+#pragma unroll
+      for (int mb = 0; mb < MB; ++mb) pipe[d][mb] = pipe[d + 1][mb];
   }
 #pragma unroll
   for (int mb = 0; mb < MB; ++mb)
@@ -612,27 +639,30 @@ static unsigned grid1d(int64_t total) {
 // Launch shape per phase: NB 16-column blocks per column group (grid z),
 // MB 16-row tiles per wave, T K-steps per double-buffered LDS stage.
 struct MfmaShape {
-  int NB1, MB1, T1;
-  int NB2, MB2, T2;
+  int NB1, MB1, T1, PD1;
+  int NB2, MB2, T2, PD2;
 };
 
 // Defaults from a sweep of every compiled shape at n = 1e5, m = 5e5 on
 // MI355X (k = 12 and 64): four-K-step LDS stages and 2-4 row tiles per wave
-// win; the phase-2 LDS budget limits three limbs to NB * T <= 5.
+// win, a second genotype super-block in flight helps phase 1 at small k; the
+// phase-2 LDS budget limits three limbs to NB * T <= 5.
 static MfmaShape default_shape(int64_t k, int L) {
-  if (k <= 16) return {1, 4, 4, 1, 4, 4};
-  if (k <= 32) return {2, 2, 4, 2, 4, 2};
-  if (L == 2) return {2, 2, 4, 4, 4, 2};
-  return {2, 2, 4, 2, 4, 2};
+  if (k <= 16) return {1, 4, 4, 2, 1, 4, 4, 1};
+  if (k <= 32) return {2, 2, 4, 1, 2, 4, 2, 1};
+  if (L == 2) return {2, 2, 4, 1, 4, 4, 2, 1};
+  return {2, 2, 4, 1, 2, 4, 2, 1};
 }
 
-// The default table, or COH_MFMA_SHAPE="nb1,mb1,t1,nb2,mb2,t2" for tuning
-// sweeps (a shape with no compiled kernel is rejected when it launches).
+// The default table, or COH_MFMA_SHAPE="nb1,mb1,t1,pd1,nb2,mb2,t2,pd2" for
+// tuning sweeps (a shape with no compiled kernel is rejected when it
+// launches).
 static MfmaShape shape_for(int64_t k, int L) {
   MfmaShape s = default_shape(k, L);
   if (const char* e = std::getenv("COH_MFMA_SHAPE")) {
     MfmaShape o;
-    if (std::sscanf(e, "%d,%d,%d,%d,%d,%d", &o.NB1, &o.MB1, &o.T1, &o.NB2, &o.MB2, &o.T2) == 6)
+    if (std::sscanf(e, "%d,%d,%d,%d,%d,%d,%d,%d", &o.NB1, &o.MB1, &o.T1, &o.PD1, &o.NB2, &o.MB2,
+                    &o.T2, &o.PD2) == 8)
       s = o;
   }
   return s;
@@ -649,14 +679,16 @@ int plan_mfma(const coh_panel* p, int precision, int64_t k, int32_t cb, int32_t 
   const int64_t nslots = pl->slot1 - pl->slot0;
   const MfmaShape sh = shape_for(k, pl->limbs);
   if ((sh.MB1 != 1 && sh.MB1 != 2 && sh.MB1 != 4) || sh.NB1 < 1 || sh.NB2 < 1 || sh.MB2 < 1 ||
-      sh.T1 < 1 || sh.T2 < 1 || 4 % sh.T1 != 0 || 4 % sh.T2 != 0)
+      sh.T1 < 1 || sh.T2 < 1 || 4 % sh.T1 != 0 || 4 % sh.T2 != 0 || sh.PD1 < 1 || sh.PD2 < 1)
     return fail(COH_ERR_VALUE, "invalid MFMA launch shape");
   pl->shape[0] = sh.NB1;
   pl->shape[1] = sh.MB1;
   pl->shape[2] = sh.T1;
-  pl->shape[3] = sh.NB2;
-  pl->shape[4] = sh.MB2;
-  pl->shape[5] = sh.T2;
+  pl->shape[3] = sh.PD1;
+  pl->shape[4] = sh.NB2;
+  pl->shape[5] = sh.MB2;
+  pl->shape[6] = sh.T2;
+  pl->shape[7] = sh.PD2;
   pl->KC = sh.NB1 * 16;
   // column blocks padded to whole groups (zero-filled)
   const int64_t NBt1 = col_blocks(k, sh.NB1), NBt2 = col_blocks(k, sh.NB2);
@@ -742,7 +774,7 @@ static int phase1(const coh_panel* p, const KvPlan& pl, const float* V, int64_t 
                   const CompScale& cs, int inv_pow, uint8_t* ws, hipStream_t st) {
   const double qmax = (L == 3) ? kQ3 : kQ2;
   const int64_t nslots = pl.slot1 - pl.slot0;
-  const int NB = pl.shape[0], MB = pl.shape[1], T = pl.shape[2];
+  const int NB = pl.shape[0], MB = pl.shape[1], T = pl.shape[2], PD = pl.shape[3];
   const int64_t NBt = col_blocks(k, NB);
   const int64_t groups = NBt / NB;
   const int64_t KB = pl.n_pad / 64;
@@ -776,21 +808,23 @@ static int phase1(const coh_panel* p, const KvPlan& pl, const float* V, int64_t 
   a.nsplit = pl.nsplit1;
   dim3 grid((unsigned)(nslots / (64 * MB)), (unsigned)pl.nsplit1, (unsigned)groups);
   char name[64];
-  snprintf(name, sizeof(name), "k_p1_mfma_L%d_NB%d_MB%d_T%d", L, NB, MB, T);
+  snprintf(name, sizeof(name), "k_p1_mfma_L%d_NB%d_MB%d_T%d_PD%d", L, NB, MB, T, PD);
   KernelTimer tm(st, name, (double)p->n * (double)ceil_div(pl.m_sel, 4),
                  2.0 * (double)p->n * (double)pl.m_sel * (double)k,
                  2.0 * L * 2.0 * (double)nslots * (double)pl.n_pad * (double)(NBt * 16));
   bool launched = false;
-#define COH_P1V(NB_, MB_, T_)                                                      \
-  if constexpr (L * NB_ * T_ <= 32)                                                \
-    if (!launched && NB == NB_ && MB == MB_ && T == T_) {                          \
-      hipLaunchKernelGGL((k_p1_mfma<L, NB_, MB_, T_>), grid, dim3(256), 0, st, a); \
-      launched = true;                                                             \
+#define COH_P1V(NB_, MB_, T_, PD_)                                                      \
+  if constexpr (L * NB_ * T_ <= 32)                                                     \
+    if (!launched && NB == NB_ && MB == MB_ && T == T_ && PD == PD_) {                  \
+      hipLaunchKernelGGL((k_p1_mfma<L, NB_, MB_, T_, PD_>), grid, dim3(256), 0, st, a); \
+      launched = true;                                                                  \
     }
-  COH_P1V(1, 4, 4) COH_P1V(1, 2, 4) COH_P1V(1, 4, 2)
-  COH_P1V(2, 2, 1) COH_P1V(2, 2, 2) COH_P1V(2, 2, 4) COH_P1V(2, 4, 1) COH_P1V(2, 4, 2)
-  COH_P1V(2, 1, 4) COH_P1V(4, 1, 1) COH_P1V(4, 1, 2) COH_P1V(4, 1, 4) COH_P1V(4, 2, 1)
-  COH_P1V(4, 2, 2) COH_P1V(4, 2, 4)
+  COH_P1V(1, 4, 4, 1) COH_P1V(1, 2, 4, 1) COH_P1V(1, 4, 2, 1)
+  COH_P1V(2, 2, 1, 1) COH_P1V(2, 2, 2, 1) COH_P1V(2, 2, 4, 1) COH_P1V(2, 4, 1, 1) COH_P1V(2, 4, 2, 1)
+  COH_P1V(2, 1, 4, 1) COH_P1V(4, 1, 1, 1) COH_P1V(4, 1, 2, 1) COH_P1V(4, 1, 4, 1) COH_P1V(4, 2, 1, 1)
+  COH_P1V(4, 2, 2, 1) COH_P1V(4, 2, 4, 1)
+  COH_P1V(1, 4, 4, 2) COH_P1V(1, 4, 4, 3) COH_P1V(1, 2, 4, 2) COH_P1V(1, 2, 4, 3)
+  COH_P1V(2, 2, 4, 2) COH_P1V(2, 2, 4, 3) COH_P1V(2, 1, 4, 2) COH_P1V(4, 1, 4, 2)
 #undef COH_P1V
   if (!launched) return fail(COH_ERR_VALUE, "no compiled phase-1 kernel for this MFMA shape");
   tm.stop();
@@ -807,7 +841,7 @@ static int phase2(const coh_panel* p, const KvPlan& pl, int64_t k, int stacked, 
                   int64_t ldo, int64_t out_stride, float beta, uint8_t* ws, hipStream_t st) {
   const double qmax = (L == 3) ? kQ3 : kQ2;
   const int64_t nslots = pl.slot1 - pl.slot0;
-  const int NB = pl.shape[3], MB = pl.shape[4], T = pl.shape[5];
+  const int NB = pl.shape[4], MB = pl.shape[5], T = pl.shape[6], PD = pl.shape[7];
   const int64_t NBt = col_blocks(k, NB);
   const int64_t groups = NBt / NB;
   const int64_t KB = nslots / 64;
@@ -844,21 +878,23 @@ static int phase2(const coh_panel* p, const KvPlan& pl, int64_t k, int stacked, 
   a.part = reinterpret_cast<int32_t*>(ws + pl.off_p1);
   dim3 grid((unsigned)ceil_div(p->n, 64 * MB), (unsigned)pl.tab.nchunks, (unsigned)groups);
   char name[64];
-  snprintf(name, sizeof(name), "k_p2_mfma_L%d_NB%d_MB%d_T%d", L, NB, MB, T);
+  snprintf(name, sizeof(name), "k_p2_mfma_L%d_NB%d_MB%d_T%d_PD%d", L, NB, MB, T, PD);
   KernelTimer tm(st, name, (double)p->n * (double)ceil_div(pl.m_sel, 4),
                  2.0 * (double)p->n * (double)pl.m_sel * (double)k,
                  2.0 * L * 2.0 * (double)nslots * (double)round_up(p->n, 64 * MB) *
                      (double)(NBt * 16));
   bool launched = false;
-#define COH_P2V(NB_, MB_, T_)                                                      \
-  if constexpr (L * NB_ * T_ <= 16)                                                \
-    if (!launched && NB == NB_ && MB == MB_ && T == T_) {                          \
-      hipLaunchKernelGGL((k_p2_mfma<L, NB_, MB_, T_>), grid, dim3(256), 0, st, a); \
-      launched = true;                                                             \
+#define COH_P2V(NB_, MB_, T_, PD_)                                                      \
+  if constexpr (L * NB_ * T_ <= 16)                                                     \
+    if (!launched && NB == NB_ && MB == MB_ && T == T_ && PD == PD_) {                  \
+      hipLaunchKernelGGL((k_p2_mfma<L, NB_, MB_, T_, PD_>), grid, dim3(256), 0, st, a); \
+      launched = true;                                                                  \
     }
-  COH_P2V(1, 8, 4) COH_P2V(1, 4, 4) COH_P2V(1, 8, 2) COH_P2V(2, 4, 1) COH_P2V(2, 4, 2)
-  COH_P2V(2, 8, 1) COH_P2V(2, 8, 2) COH_P2V(4, 2, 1) COH_P2V(4, 2, 2) COH_P2V(4, 4, 1)
-  COH_P2V(4, 4, 2) COH_P2V(4, 1, 2)
+  COH_P2V(1, 8, 4, 1) COH_P2V(1, 4, 4, 1) COH_P2V(1, 8, 2, 1) COH_P2V(2, 4, 1, 1) COH_P2V(2, 4, 2, 1)
+  COH_P2V(2, 8, 1, 1) COH_P2V(2, 8, 2, 1) COH_P2V(4, 2, 1, 1) COH_P2V(4, 2, 2, 1) COH_P2V(4, 4, 1, 1)
+  COH_P2V(4, 4, 2, 1) COH_P2V(4, 1, 2, 1)
+  COH_P2V(1, 4, 4, 2) COH_P2V(1, 4, 4, 3) COH_P2V(1, 8, 4, 2) COH_P2V(2, 4, 2, 2)
+  COH_P2V(2, 4, 2, 3) COH_P2V(4, 4, 2, 2) COH_P2V(4, 2, 2, 2)
 #undef COH_P2V
   if (!launched) return fail(COH_ERR_VALUE, "no compiled phase-2 kernel for this MFMA shape");
   tm.stop();

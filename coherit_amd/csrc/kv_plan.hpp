@@ -52,7 +52,7 @@ struct KvPlan {
   int limbs;
   int64_t n_pad;            // samples padded to the MFMA K tile
   int64_t m_sel;            // real SNPs of the requested components
-  int shape[6];             // MFMA launch shape: NB, MB, T of phase 1 then phase 2
+  int shape[8];             // MFMA launch shape: NB, MB, T, PD of phase 1 then phase 2
 };
 
 inline int64_t selected_snps(const coh_panel* p, int32_t cb, int32_t ce) {
