@@ -1,0 +1,234 @@
+// Re-orthogonalisation primitives of the batched SLQ Lanczos (slq.py:49-74)
+// and of its reverse sweep (slq.py:77-81, 124-140).
+//
+// Vectors are probe-major: basis[s][j][i] = Lanczos vector j of probe s
+// (element stride 1, vector stride `vs`, probe stride `ps`), and a block of
+// per-probe vectors X[s][i] has row stride ldx.  Each step of the two-pass
+// full re-orthogonalisation is
+//     c[s][j] = sum_i P[s][j][i] X[s][i]          (coh_probe_dots)
+//     Y[s][i] = X[s][i] - sum_j P[s][j][i] c[s][j] (coh_probe_project_out)
+// and the reverse sweep adds a rank-q update to the basis cotangent
+//     B[s][j][i] += sum_q coeff[s][j][q] V[s][q][i] (coh_probe_rank_update).
+// These are GEMV-shaped (r <= a few hundred, n ~ 1e5..1e6) and bound by
+// streaming the basis from HBM: one block per (probe, 2048-element chunk),
+// coalesced along i.  Dot products accumulate FP32 products in FP64 with a
+// fixed two-stage order (deterministic).
+#include "common.hpp"
+
+#include <algorithm>
+
+namespace coh {
+
+constexpr int kEPT = 8;               // elements per thread
+constexpr int kChunk = 256 * kEPT;    // elements per block
+constexpr int kJT = 8;                // basis vectors per register tile
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+// part[(s * r + j) * nch + ch] = sum over chunk ch of P[s][j][i] X[s][i]
+__global__ __launch_bounds__(256) void k_probe_dots_part(int64_t n, int r, const float* __restrict__ P,
+                                                         int64_t ps, int64_t vs,
+                                                         const float* __restrict__ X, int64_t ldx,
+                                                         double* __restrict__ part, int nch) {
+  const int s = blockIdx.y, ch = blockIdx.x, t = threadIdx.x;
+  const int wave = t / 64, lane = t % 64;
+  const int64_t i0 = (int64_t)ch * kChunk + t;
+  float x[kEPT];
+#pragma unroll
+  for (int e = 0; e < kEPT; ++e) {
+    const int64_t i = i0 + 256 * e;
+    x[e] = (i < n) ? X[(int64_t)s * ldx + i] : 0.0f;
+  }
+  const float* Ps = P + (int64_t)s * ps;
+  __shared__ double red[kJT][4];
+  for (int j0 = 0; j0 < r; j0 += kJT) {
+    double acc[kJT];
+#pragma unroll
+    for (int jj = 0; jj < kJT; ++jj) {
+      acc[jj] = 0.0;
+      const int j = j0 + jj;
+      if (j < r) {
+#pragma unroll
+        for (int e = 0; e < kEPT; ++e) {
+          const int64_t i = i0 + 256 * e;
+          if (i < n) acc[jj] += static_cast<double>(Ps[(int64_t)j * vs + i]) * static_cast<double>(x[e]);
+        }
+      }
+    }
+#pragma unroll
+    for (int jj = 0; jj < kJT; ++jj) {
+      const double v = wave_sum(acc[jj]);
+      if (lane == 0) red[jj][wave] = v;
+    }
+    __syncthreads();
+    if (t < kJT && j0 + t < r)
+      part[((int64_t)s * r + j0 + t) * nch + ch] = red[t][0] + red[t][1] + red[t][2] + red[t][3];
+    __syncthreads();
+  }
+}
+
+__global__ void k_probe_dots_final(int64_t S, int r, int nch, const double* __restrict__ part,
+                                   float* __restrict__ C, int64_t ldc) {
+  const int64_t total = S * r;
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t s = idx / r;
+    const int j = static_cast<int>(idx % r);
+    double sum = 0.0;
+    for (int ch = 0; ch < nch; ++ch) sum += part[idx * nch + ch];
+    C[s * ldc + j] = static_cast<float>(sum);
+  }
+}
+
+// Y[s][i] = X[s][i] - sum_j P[s][j][i] C[s][j]; Y may alias X.
+__global__ __launch_bounds__(256) void k_probe_project_out(int64_t n, int r, const float* __restrict__ P,
+                                                           int64_t ps, int64_t vs,
+                                                           const float* __restrict__ C, int64_t ldc,
+                                                           const float* X, int64_t ldx, float* Y,
+                                                           int64_t ldy) {
+  const int s = blockIdx.y;
+  const int64_t i0 = (int64_t)blockIdx.x * kChunk + threadIdx.x;
+  const float* Ps = P + (int64_t)s * ps;
+  const float* Cs = C + (int64_t)s * ldc;
+  float acc[kEPT];
+#pragma unroll
+  for (int e = 0; e < kEPT; ++e) acc[e] = 0.0f;
+  for (int j = 0; j < r; ++j) {
+    const float cj = Cs[j];
+    const float* Pj = Ps + (int64_t)j * vs;
+#pragma unroll
+    for (int e = 0; e < kEPT; ++e) {
+      const int64_t i = i0 + 256 * e;
+      if (i < n) acc[e] = fmaf(Pj[i], cj, acc[e]);
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < kEPT; ++e) {
+    const int64_t i = i0 + 256 * e;
+    if (i < n) Y[(int64_t)s * ldy + i] = X[(int64_t)s * ldx + i] - acc[e];
+  }
+}
+
+// B[s][j][i] += sum_q coeff[s][j][q] V[s][q][i] for j < r, q < Q <= 4.
+__global__ __launch_bounds__(256) void k_probe_rank_update(int64_t n, int r, int Q,
+                                                           const float* __restrict__ coeff,
+                                                           int64_t cps, int64_t cvs,
+                                                           const float* __restrict__ V, int64_t vps,
+                                                           int64_t vvs, float* __restrict__ B,
+                                                           int64_t bps, int64_t bvs) {
+  const int s = blockIdx.y;
+  const int64_t i0 = (int64_t)blockIdx.x * kChunk + threadIdx.x;
+  float v[4][kEPT];
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int e = 0; e < kEPT; ++e) {
+      const int64_t i = i0 + 256 * e;
+      v[q][e] = (q < Q && i < n) ? V[(int64_t)s * vps + (int64_t)q * vvs + i] : 0.0f;
+    }
+  float* Bs = B + (int64_t)s * bps;
+  const float* Cs = coeff + (int64_t)s * cps;
+  for (int j = 0; j < r; ++j) {
+    float c[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) c[q] = (q < Q) ? Cs[(int64_t)j * cvs + q] : 0.0f;
+    float* Bj = Bs + (int64_t)j * bvs;
+#pragma unroll
+    for (int e = 0; e < kEPT; ++e) {
+      const int64_t i = i0 + 256 * e;
+      if (i < n) {
+        float add = 0.0f;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) add = fmaf(c[q], v[q][e], add);
+        Bj[i] += add;
+      }
+    }
+  }
+}
+
+static int check_dims(int64_t S, int64_t r, int64_t n) {
+  if (S < 0 || r < 0 || n < 0) return fail(COH_ERR_VALUE, "negative dimension");
+  if (S > 65535) return fail(COH_ERR_VALUE, "at most 65535 probes per call");
+  if (r > (1 << 20)) return fail(COH_ERR_VALUE, "too many basis vectors");
+  return COH_OK;
+}
+
+}  // namespace coh
+
+using namespace coh;
+
+extern "C" int coh_probe_dots_workspace_bytes(int64_t S, int64_t r, int64_t n, uint64_t* bytes) {
+  COH_REQUIRE(bytes != nullptr, COH_ERR_VALUE, "bytes is NULL");
+  int st = check_dims(S, r, n);
+  if (st != COH_OK) return st;
+  const int64_t nch = std::max<int64_t>(1, ceil_div(n, kChunk));
+  *bytes = (uint64_t)round_up(std::max<int64_t>(1, S * r * nch) * 8, 256);
+  return COH_OK;
+}
+
+extern "C" int coh_probe_dots(int64_t S, int64_t r, int64_t n, const float* P, int64_t ps,
+                              int64_t vs, const float* X, int64_t ldx, float* C, int64_t ldc,
+                              void* ws, uint64_t ws_bytes, void* stream) {
+  int st = check_dims(S, r, n);
+  if (st != COH_OK) return st;
+  if (S == 0 || r == 0) return COH_OK;
+  COH_REQUIRE(P && X && C && ws, COH_ERR_VALUE, "NULL pointer argument");
+  COH_REQUIRE(ldx >= n && ldc >= r && vs >= n && ps >= r * vs, COH_ERR_VALUE,
+              "inconsistent strides");
+  uint64_t need = 0;
+  coh_probe_dots_workspace_bytes(S, r, n, &need);
+  COH_REQUIRE(ws_bytes >= need, COH_ERR_VALUE, "workspace too small");
+  const int nch = (int)std::max<int64_t>(1, ceil_div(n, kChunk));
+  hipStream_t hs = static_cast<hipStream_t>(stream);
+  double* part = static_cast<double*>(ws);
+  hipLaunchKernelGGL(k_probe_dots_part, dim3(nch, (unsigned)S), dim3(256), 0, hs, n, (int)r, P, ps,
+                     vs, X, ldx, part, nch);
+  COH_CHECK_LAUNCH();
+  const int64_t total = S * r;
+  const unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(total, 256), 4096));
+  hipLaunchKernelGGL(k_probe_dots_final, dim3(blocks), dim3(256), 0, hs, S, (int)r, nch, part, C,
+                     ldc);
+  COH_CHECK_LAUNCH();
+  return COH_OK;
+}
+
+extern "C" int coh_probe_project_out(int64_t S, int64_t r, int64_t n, const float* P, int64_t ps,
+                                     int64_t vs, const float* C, int64_t ldc, const float* X,
+                                     int64_t ldx, float* Y, int64_t ldy, void* stream) {
+  int st = check_dims(S, r, n);
+  if (st != COH_OK) return st;
+  if (S == 0 || n == 0) return COH_OK;
+  COH_REQUIRE(X && Y && (r == 0 || (P && C)), COH_ERR_VALUE, "NULL pointer argument");
+  COH_REQUIRE(ldx >= n && ldy >= n && (r == 0 || (ldc >= r && vs >= n && ps >= r * vs)),
+              COH_ERR_VALUE, "inconsistent strides");
+  const int nch = (int)std::max<int64_t>(1, ceil_div(n, kChunk));
+  hipLaunchKernelGGL(k_probe_project_out, dim3(nch, (unsigned)S), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), n, (int)r, P, ps, vs, C, ldc, X, ldx, Y,
+                     ldy);
+  COH_CHECK_LAUNCH();
+  return COH_OK;
+}
+
+extern "C" int coh_probe_rank_update(int64_t S, int64_t r, int64_t n, int32_t q,
+                                     const float* coeff, int64_t cps, int64_t cvs, const float* V,
+                                     int64_t vps, int64_t vvs, float* B, int64_t bps, int64_t bvs,
+                                     void* stream) {
+  int st = check_dims(S, r, n);
+  if (st != COH_OK) return st;
+  COH_REQUIRE(q >= 1 && q <= 4, COH_ERR_VALUE, "rank update takes 1 to 4 vectors");
+  if (S == 0 || r == 0 || n == 0) return COH_OK;
+  COH_REQUIRE(coeff && V && B, COH_ERR_VALUE, "NULL pointer argument");
+  COH_REQUIRE(cvs >= q && cps >= r * cvs && vvs >= n && vps >= q * vvs && bvs >= n &&
+                  bps >= r * bvs,
+              COH_ERR_VALUE, "inconsistent strides");
+  const int nch = (int)std::max<int64_t>(1, ceil_div(n, kChunk));
+  hipLaunchKernelGGL(k_probe_rank_update, dim3(nch, (unsigned)S), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), n, (int)r, (int)q, coeff, cps, cvs, V, vps,
+                     vvs, B, bps, bvs);
+  COH_CHECK_LAUNCH();
+  return COH_OK;
+}

@@ -12,8 +12,12 @@ so the re-orthogonalisation is a batched GEMM over probes.
 """
 from __future__ import annotations
 
+import ctypes
+
 import numpy as np
 import torch
+
+from . import _native as N
 
 EPS32 = float(np.finfo(np.float32).eps)
 
@@ -53,8 +57,51 @@ def rademacher_probes(seed: int, n: int, nsamples: int) -> np.ndarray:
     return (rng.integers(0, 2, size=(int(nsamples), int(n)), dtype=np.int8) * 2 - 1).astype(np.float32)
 
 
+def _on_device(P, w) -> bool:
+    """FP32 device tensors run the HIP re-orthogonalisation kernels; other
+    tensors (the FP64 CPU adjoint checks) run the same algebra in torch."""
+    return P.is_cuda and P.dtype == torch.float32 and w.dtype == torch.float32
+
+
+_DOTS_WS: dict = {}
+
+
+def _probe_dots(P, x):
+    """c[s, j] = sum_i P[s, j, i] x[s, i] (coh_probe_dots)."""
+    S, r, n = P.shape
+    x = x if x.is_contiguous() else x.contiguous()
+    c = torch.empty((S, r), dtype=torch.float32, device=P.device)
+    nbytes = ctypes.c_uint64()
+    N.call("coh_probe_dots_workspace_bytes", S, r, n, ctypes.byref(nbytes))
+    key = (P.device, int(nbytes.value))
+    ws = _DOTS_WS.get(key)
+    if ws is None:
+        _DOTS_WS.clear()
+        ws = torch.empty(max(int(nbytes.value), 256), dtype=torch.uint8, device=P.device)
+        _DOTS_WS[key] = ws
+    N.call("coh_probe_dots", S, r, n, N.ptr(P), P.stride(0), P.stride(1), N.ptr(x), x.stride(0),
+           N.ptr(c), c.stride(0), N.ptr(ws), int(nbytes.value), N.stream_ptr())
+    return c
+
+
+def _project_out(P, c, x):
+    """x - sum_j P[:, j, :] c[:, j] (coh_probe_project_out)."""
+    S, r, n = P.shape
+    x = x if x.is_contiguous() else x.contiguous()
+    y = torch.empty_like(x)
+    N.call("coh_probe_project_out", S, r, n, N.ptr(P), P.stride(0), P.stride(1), N.ptr(c),
+           c.stride(0), N.ptr(x), x.stride(0), N.ptr(y), y.stride(0), N.stream_ptr())
+    return y
+
+
 def _reorth(P, w):
-    """Two passes of w -= P (P^T w) per probe; returns (w1.., coeffs, inputs)."""
+    """Two passes of w -= P (P^T w) per probe; returns (c1, w2, c2, w3)."""
+    if _on_device(P, w):
+        c1 = _probe_dots(P, w)
+        w2 = _project_out(P, c1, w)
+        c2 = _probe_dots(P, w2)
+        w3 = _project_out(P, c2, w2)
+        return c1, w2, c2, w3
     c1 = torch.bmm(P, w[:, :, None])[:, :, 0]          # (S, r)
     w2 = w - torch.bmm(P.transpose(1, 2), c1[:, :, None])[:, :, 0]
     c2 = torch.bmm(P, w2[:, :, None])[:, :, 0]
@@ -103,15 +150,28 @@ def _step_pullback(basis, product, previous_beta, index, q_bar_next, alpha_bar, 
     w3b = torch.where(keep, (q_bar_next - nxt * proj) / norm + beta_bar[:, None] * nxt,
                       torch.zeros_like(nxt))
     P = s["P"]
-    # pass 2: w3 = w2 - P c2, c2 = P^T w2
-    c2b = -torch.bmm(P, w3b[:, :, None])[:, :, 0]
-    w2b = w3b + torch.bmm(P.transpose(1, 2), c2b[:, :, None])[:, :, 0]
-    # pass 1: w2 = w1 - P c1, c1 = P^T w1
-    c1b = -torch.bmm(P, w2b[:, :, None])[:, :, 0]
-    w1b = w2b + torch.bmm(P.transpose(1, 2), c1b[:, :, None])[:, :, 0]
-    coeffs = torch.stack([s["c2"], c2b, s["c1"], c1b], dim=2)          # (S, r, 4)
-    vecs = torch.stack([-w3b, s["w2"], -w2b, s["w1"]], dim=1)          # (S, 4, n)
-    basis_bar[:, : index + 1, :] += torch.bmm(coeffs, vecs)
+    if _on_device(P, w3b):
+        # pass 2: w3 = w2 - P c2, c2 = P^T w2;  pass 1 likewise
+        d2 = _probe_dots(P, w3b)
+        w2b = _project_out(P, d2, w3b)
+        d1 = _probe_dots(P, w2b)
+        w1b = _project_out(P, d1, w2b)
+        c2b, c1b = -d2, -d1
+        coeffs = torch.stack([s["c2"], c2b, s["c1"], c1b], dim=2).contiguous()   # (S, r, 4)
+        vecs = torch.stack([-w3b, s["w2"], -w2b, s["w1"]], dim=1).contiguous()   # (S, 4, n)
+        bb = basis_bar[:, : index + 1, :]
+        S_, r_, n_ = P.shape
+        N.call("coh_probe_rank_update", S_, r_, n_, 4, N.ptr(coeffs), coeffs.stride(0),
+               coeffs.stride(1), N.ptr(vecs), vecs.stride(0), vecs.stride(1), N.ptr(bb),
+               bb.stride(0), bb.stride(1), N.stream_ptr())
+    else:
+        c2b = -torch.bmm(P, w3b[:, :, None])[:, :, 0]
+        w2b = w3b + torch.bmm(P.transpose(1, 2), c2b[:, :, None])[:, :, 0]
+        c1b = -torch.bmm(P, w2b[:, :, None])[:, :, 0]
+        w1b = w2b + torch.bmm(P.transpose(1, 2), c1b[:, :, None])[:, :, 0]
+        coeffs = torch.stack([s["c2"], c2b, s["c1"], c1b], dim=2)          # (S, r, 4)
+        vecs = torch.stack([-w3b, s["w2"], -w2b, s["w1"]], dim=1)          # (S, 4, n)
+        basis_bar[:, : index + 1, :] += torch.bmm(coeffs, vecs)
     # w1 = w0 - alpha q, alpha = q . w0
     q = s["q"]
     a_tot = torch.where(s["active"], alpha_bar, torch.zeros_like(alpha_bar)) - (q * w1b).sum(1)

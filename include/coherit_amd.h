@@ -197,6 +197,34 @@ int coh_lanczos_step(int64_t n, int64_t k, const float* dev_v, const float* dev_
                      uint64_t workspace_bytes, void* stream);
 
 /* ---------------------------------------------------------------------
+ * Batched Lanczos re-orthogonalisation (slq.py:49-74 two-pass full
+ * re-orthogonalisation; its reverse sweep slq.py:77-81, 124-140).
+ * Probe-major layout: P[s][j][i] at P + s*ps + j*vs + i (vs >= n,
+ * ps >= r*vs); per-probe vectors X[s][i] at X + s*ldx + i.
+ * ------------------------------------------------------------------- */
+
+/* Workspace bytes of coh_probe_dots for S probes, r vectors, length n. */
+int coh_probe_dots_workspace_bytes(int64_t S, int64_t r, int64_t n, uint64_t* host_bytes);
+/* C[s][j] = sum_i P[s][j][i] X[s][i] (C at C + s*ldc + j), FP32 products
+ * accumulated in FP64 in a fixed order; replaces the coefficient matmul
+ * P^T w of one re-orthogonalisation pass (slq.py:60-64). */
+int coh_probe_dots(int64_t S, int64_t r, int64_t n, const float* dev_P, int64_t ps, int64_t vs,
+                   const float* dev_X, int64_t ldx, float* dev_C, int64_t ldc,
+                   void* dev_workspace, uint64_t workspace_bytes, void* stream);
+/* Y[s][i] = X[s][i] - sum_j P[s][j][i] C[s][j]; Y may alias X.  The update
+ * w - P c of one re-orthogonalisation pass (slq.py:60-64). */
+int coh_probe_project_out(int64_t S, int64_t r, int64_t n, const float* dev_P, int64_t ps,
+                          int64_t vs, const float* dev_C, int64_t ldc, const float* dev_X,
+                          int64_t ldx, float* dev_Y, int64_t ldy, void* stream);
+/* B[s][j][i] += sum_{q<Q} coeff[s][j][q] V[s][q][i] for j < r, 1 <= Q <= 4
+ * (coeff at coeff + s*cps + j*cvs + q; V and B probe-major like P): the
+ * basis cotangent of both re-orthogonalisation passes in the reverse sweep
+ * (the VJP of slq.py:60-64 taken by slq.py:77-81). */
+int coh_probe_rank_update(int64_t S, int64_t r, int64_t n, int32_t Q, const float* dev_coeff,
+                          int64_t cps, int64_t cvs, const float* dev_V, int64_t vps, int64_t vvs,
+                          float* dev_B, int64_t bps, int64_t bvs, void* stream);
+
+/* ---------------------------------------------------------------------
  * Kernel timing instrumentation (the reference only logs wall-clock
  * per-iteration fields, reml.py:1979-1980).  When enabled, each main
  * contraction kernel launch is bracketed by HIP events on its stream and

@@ -179,3 +179,38 @@ def test_slq_value_and_gradient_do_not_depend_on_probe_batching():
     for v, g in out[1:]:
         assert float(v) == pytest.approx(float(out[0][0]), rel=1e-5)
         np.testing.assert_allclose(np.asarray(g), np.asarray(out[0][1]), rtol=1e-4)
+
+
+def test_reorth_primitives_match_fp64_products():
+    """coh_probe_dots / _project_out / _rank_update against FP64 torch on the
+    device, on a probe-major basis view with depth padding (slq.py:49-74)."""
+    from coherit_amd import slq
+
+    torch.manual_seed(0)
+    S, D, r, n = 5, 9, 6, 5000 + 37
+    basis = torch.randn(S, D, n, device="cuda")
+    P = basis[:, :r, :]
+    x = torch.randn(S, n, device="cuda")
+    c = slq._probe_dots(P, x)
+    ref = torch.einsum("sjn,sn->sj", P.double(), x.double())
+    torch.testing.assert_close(c.double(), ref, rtol=1e-6, atol=1e-4)
+    y = slq._project_out(P, c, x)
+    ref_y = x.double() - torch.einsum("sjn,sj->sn", P.double(), c.double())
+    torch.testing.assert_close(y.double(), ref_y, rtol=1e-5, atol=1e-4)
+    bb = torch.zeros(S, D, n, device="cuda")
+    coeff = torch.randn(S, r, 4, device="cuda")
+    vecs = torch.randn(S, 4, n, device="cuda")
+    from coherit_amd import _native as N
+    view = bb[:, :r, :]
+    N.call("coh_probe_rank_update", S, r, n, 4, N.ptr(coeff), coeff.stride(0), coeff.stride(1),
+           N.ptr(vecs), vecs.stride(0), vecs.stride(1), N.ptr(view), view.stride(0),
+           view.stride(1), N.stream_ptr())
+    ref_b = torch.einsum("sjq,sqn->sjn", coeff.double(), vecs.double())
+    torch.testing.assert_close(bb[:, :r, :].double(), ref_b, rtol=1e-5, atol=1e-5)
+    assert float(bb[:, r:, :].abs().max()) == 0.0
+    # the two-pass step equals the torch formulation on the same inputs
+    c1, w2, c2, w3 = slq._reorth(P, x)
+    Pd, xd = P.double(), x.double()
+    c1r = torch.bmm(Pd, xd[:, :, None])[:, :, 0]
+    w2r = xd - torch.bmm(Pd.transpose(1, 2), c1r[:, :, None])[:, :, 0]
+    torch.testing.assert_close(w2.double(), w2r, rtol=1e-5, atol=1e-3)
