@@ -597,6 +597,16 @@ extern "C" int coh_panel_copy_stats(const coh_panel* p, float* om, float* oi, in
   return COH_OK;
 }
 
+extern "C" int coh_panel_set_standardization(coh_panel* p, const float* mo, const float* io,
+                                             void* stream) {
+  COH_REQUIRE(p && mo && io, COH_ERR_VALUE, "NULL argument");
+  COH_REQUIRE(p->finalized, COH_ERR_RUNTIME, "panel is not finalized");
+  hipLaunchKernelGGL(k_override, dim3(256), dim3(256), 0, (hipStream_t)stream, mo, io, p->m,
+                     p->cache_to_slot, p->mean, p->inv);
+  COH_CHECK_LAUNCH();
+  return COH_OK;
+}
+
 extern "C" int coh_panel_read_packed_call(const coh_panel* p, int32_t c, uint8_t* out, int64_t ld,
                                           void* stream) {
   COH_REQUIRE(p && out, COH_ERR_VALUE, "NULL argument");

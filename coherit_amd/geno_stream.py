@@ -337,6 +337,17 @@ class GenoBlockStreamer:
             for c in range(self._n_calls)
         )
 
+    def copy_standardization_from(self, other: "GenoBlockStreamer") -> None:
+        """Standardize this panel with another panel's mean / inv_sd
+        (reml_model.py:275-279); operator scales (eff) are unchanged."""
+        if other.m != self.m:
+            raise ValueError(f"SNP count mismatch: expected {self.m}, got {other.m}.")
+        mean, inv = other._panel_stats_host()
+        self._panel.set_standardization(mean, inv)
+        if self._means_host is not None:
+            self._means_host = np.asarray(mean, np.float32).copy()
+            self._inv_sds_host = np.asarray(inv, np.float32).copy()
+
     # ---------------------------------------------------- compat accessors
     @property
     def _means_by_call(self):

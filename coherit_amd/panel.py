@@ -134,6 +134,15 @@ class ResidentPanel:
                N.ptr(s2), N.stream_ptr())
         return tuple(t.cpu().numpy() for t in (mean, inv, cnt, s1, s2))
 
+    def set_standardization(self, mean, inv_sd) -> None:
+        """Replace mean / inv_sd (cache order, m entries) of the finalized panel."""
+        mo = torch.as_tensor(np.asarray(mean, np.float32).reshape(-1), device=self.device)
+        io = torch.as_tensor(np.asarray(inv_sd, np.float32).reshape(-1), device=self.device)
+        if mo.numel() != self.m or io.numel() != self.m:
+            raise ValueError(f"standardization needs {self.m} entries per array")
+        N.call("coh_panel_set_standardization", self.handle, N.ptr(mo), N.ptr(io), N.stream_ptr())
+        torch.cuda.current_stream(self.device).synchronize()
+
     def read_packed_call(self, c: int) -> torch.Tensor:
         pw = (int(self.call_widths[c]) + 3) // 4
         out = torch.empty((self.n, pw), dtype=torch.uint8, device=self.device)

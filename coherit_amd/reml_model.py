@@ -17,8 +17,10 @@ import numpy as np
 import torch
 
 from .geno_stream import GenoBlockStreamer
-from .precond import ProjectedCorePrecondConf, build_lowrank_basis
-from .reml import REMLProbeCache, fit_reml
+from .pcg import pcg_solve
+from .precond import (ProjectedCorePrecondConf, build_lowrank_basis, make_precond,
+                      scalar_diag_from_precond_conf)
+from .reml import REMLProbeCache, fit_reml, standardize_response
 from .slq import default_workspace_bytes
 
 logger = logging.getLogger(__name__)
@@ -71,6 +73,7 @@ class FitResult:
     var_components: np.ndarray
     history: list
     rep_var_components: Optional[np.ndarray] = None
+    effects: Optional["EffectEstimates"] = None
     final_ai: Optional[np.ndarray] = None
     final_grad: Optional[np.ndarray] = None
     final_loglik: Optional[float] = None
@@ -81,6 +84,51 @@ class FitResult:
     monte_carlo_se_var: Optional[np.ndarray] = None
     monte_carlo_se_h2: Optional[float] = None
     counters: Optional[dict] = None
+
+
+@dataclasses.dataclass
+class EffectEstimates:
+    """reml_model.py:86-97: effects on the internally standardized y scale."""
+
+    fixed_effects: torch.Tensor
+    random_effect: torch.Tensor
+    random_effect_components: tuple
+    snp_effects: tuple
+    pcg_rel_res: float
+    pcg_iters: int
+    y_mean: float
+    y_scale: float
+
+
+@dataclasses.dataclass
+class PredictionEstimates:
+    """reml_model.py:100-108."""
+
+    fixed_effect: torch.Tensor
+    random_effect: torch.Tensor
+    random_effect_components: tuple
+    y_pred_std: torch.Tensor
+    y_pred: torch.Tensor
+
+
+def _solve_small_spd(A, b, *, base_jitter: float = 1e-6, max_tries: int = 8) -> np.ndarray:
+    """reml_model.py:296-327: host FP64 Cholesky with a jitter ladder."""
+    A = np.asarray(A, np.float64)
+    A = 0.5 * (A + A.T)
+    b = np.asarray(b, np.float64).reshape(-1)
+    p = A.shape[0]
+    if p == 0:
+        raise ValueError("solve_small_spd requires a non-empty square matrix.")
+    scale = float(np.mean(np.abs(np.diag(A))))
+    if not np.isfinite(scale) or scale <= 0.0:
+        scale = 1.0
+    for k in range(max_tries):
+        try:
+            L = np.linalg.cholesky(A + base_jitter * scale * (10.0 ** k) * np.eye(p))
+        except np.linalg.LinAlgError:
+            continue
+        return np.linalg.solve(L.T, np.linalg.solve(L, b)).astype(np.float32)
+    raise FloatingPointError("Failed to stabilize GLS normal equations with jitter escalation.")
 
 
 @dataclasses.dataclass(frozen=True)
@@ -270,8 +318,6 @@ class InfinitesimalREMLFitter:
     def fit_infinitesimal(self, y, covar=None, h2_init: float = 0.5, var_components_init=None,
                           estimate_effects: bool = False, counters: dict | None = None) -> FitResult:
         """reml_model.py:1836-2010."""
-        if estimate_effects:
-            raise NotImplementedError("Effect estimation is not part of this engine yet.")
         cfg = self.cfg
         ops = self._assemble_reml_operators()
         if cfg.precond_rank > 0 and self.precond_conf is None:
@@ -324,8 +370,13 @@ class InfinitesimalREMLFitter:
             se_h2 = float(np.sqrt(max(0.0, ((hv - hv.mean()) ** 2).sum() / (R * (R - 1)))))
             rep_vc = stack
             diagnostics = None
+        effects = None
+        if estimate_effects:
+            # reml_model.py:1981-1986
+            effects = self.estimate_effects(y, var_components=vc_mean, covar=covar)
         return FitResult(
             var_components=np.asarray(vc_mean), history=history, rep_var_components=rep_vc,
+            effects=effects,
             final_ai=None if diagnostics is None else diagnostics.get("ai"),
             final_grad=None if diagnostics is None else diagnostics.get("grad"),
             final_loglik=None if diagnostics is None else float(diagnostics.get("loglik")),
@@ -335,9 +386,186 @@ class InfinitesimalREMLFitter:
             counters=counters,
         )
 
+    # ----------------------------------------------------- effects/predict
+    def estimate_effects(self, y, *, var_components, covar=None) -> EffectEstimates:
+        """reml_model.py:1544-1647: GLS fixed effects, BLUP random effects and
+        per-component SNP effects at the given variance components."""
+        if not self.streamers:
+            raise RuntimeError("estimate_effects requires initialized streamers.")
+        cfg = self.cfg
+        ops = self._assemble_reml_operators()
+        theta = np.asarray(var_components, np.float32).astype(np.float64).reshape(-1)
+        G = len(ops.K_mvs)
+        if theta.size != G + 1:
+            raise ValueError(f"var_components length mismatch: expected {G + 1}, got {theta.size}.")
+        theta_g, theta_e = theta[:-1], float(theta[-1])
+        if cfg.precond_rank > 0 and self.precond_conf is None:
+            self._build_projected_core_precond(ops, theta.astype(np.float32))
+        dev = torch.device("cuda", torch.cuda.current_device())
+        if cfg.response_is_standardized:
+            y_std = torch.as_tensor(np.asarray(y, np.float32).reshape(-1), device=dev)
+            if not bool(torch.isfinite(y_std).all()):
+                raise ValueError("Phenotype contains non-finite values.")
+            y_mean, y_scale = 0.0, 1.0
+        else:
+            y_std, y_mean, y_scale = standardize_response(y)
+            y_std = torch.as_tensor(y_std, dtype=torch.float32, device=dev).reshape(-1)
+            y_mean, y_scale = float(y_mean), float(y_scale)
+        xmat = None
+        if covar is not None:
+            xmat = torch.as_tensor(np.asarray(covar, np.float32), device=dev)
+            if xmat.ndim == 1:
+                xmat = xmat[:, None]
+            if xmat.numel() == 0:
+                xmat = None
+
+        # reml_model.py:1431-1441 (_make_hv), 1443-1457 (_make_effect_precond)
+        if ops.weighted_hv is not None:
+            Hv = lambda V: ops.weighted_hv(theta_g, theta_e, V)
+        else:
+            def Hv(V):
+                acc = theta_e * V
+                for g, mv in enumerate(ops.K_mvs):
+                    acc = acc + float(theta_g[g]) * mv(V)
+                return acc
+        Hv_true = None
+        if ops.weighted_hv_true is not None:
+            Hv_true = lambda V: ops.weighted_hv_true(theta_g, theta_e, V)
+        M_cur = None
+        if self.precond_conf is not None:
+            diag_H = scalar_diag_from_precond_conf(self.precond_conf, theta_g, [theta_e])
+            if diag_H is None:
+                diag_H = theta_e + float(np.dot(theta_g, self._diag_atoms(ops.diag_list)))
+            M_cur = make_precond(self.precond_conf, theta_g, diag_H, eps=cfg.pcg_ridge)
+
+        rhs = y_std[:, None] if xmat is None else torch.cat([y_std[:, None], xmat], dim=1)
+        rhs = rhs.contiguous()
+        X0 = M_cur(rhs) if M_cur is not None else torch.zeros_like(rhs)
+        sol, rel_res, iters = pcg_solve(Hv, rhs, M=M_cur, tol=cfg.effect_pcg_tol,
+                                        maxiter=cfg.max_pcg_iters, X0=X0, Hv_true=Hv_true)
+        if not bool(torch.isfinite(sol).all()):
+            raise FloatingPointError("Non-finite PCG solution encountered in estimate_effects.")
+        if not np.isfinite(rel_res) or rel_res > float(cfg.effect_pcg_tol) * 1.05:
+            raise FloatingPointError(
+                "Effect-estimation PCG did not converge: "
+                f"relative residual={rel_res:.3e}, tolerance={float(cfg.effect_pcg_tol):.3e}, "
+                f"iterations={int(iters)}/{int(cfg.max_pcg_iters)}.")
+        Hinv_y = sol[:, 0]
+        if xmat is not None:
+            Hinv_x = sol[:, 1:]
+            XtHinvX = (xmat.T @ Hinv_x).double().cpu().numpy()
+            beta = _solve_small_spd(XtHinvX, (xmat.T @ Hinv_y).double().cpu().numpy())
+            fixed = torch.as_tensor(beta, device=dev)
+            alpha = Hinv_y - Hinv_x @ fixed
+        else:
+            fixed = torch.empty(0, dtype=torch.float32, device=dev)
+            alpha = Hinv_y
+        alpha = alpha.contiguous()
+        if ops.stacked_kv is not None:
+            Kalpha = ops.stacked_kv(alpha[:, None])[:, :, 0]
+        else:
+            Kalpha = torch.stack([mv(alpha) for mv in ops.K_mvs])
+        comps = tuple(float(theta_g[g]) * Kalpha[g] for g in range(G))
+        random_effect = sum(comps[1:], comps[0])
+        snp = self._estimate_snp_effects(alpha, theta_g)
+        return EffectEstimates(fixed_effects=fixed, random_effect=random_effect,
+                               random_effect_components=comps, snp_effects=snp,
+                               pcg_rel_res=float(rel_res), pcg_iters=int(iters), y_mean=y_mean,
+                               y_scale=y_scale)
+
+    def _estimate_snp_effects(self, alpha, theta_g):
+        """reml_model.py:1459-1542 (dense branches): theta_g/eff_g Z_g^T alpha."""
+        if self._partitioned_streamer is not None:
+            st = self._partitioned_streamer
+            xt_alpha = st.xtv(alpha, normalize=False)
+            out = []
+            for g in range(st.n_components):
+                a = int(st._component_snp_offsets[g])
+                b = int(st._component_snp_offsets[g + 1])
+                eff = float(st._component_eff_m_host[g])
+                if eff > 0.0:
+                    out.append(float(theta_g[g]) / eff * xt_alpha[a:b].reshape(-1))
+                else:
+                    out.append(torch.zeros(b - a, dtype=xt_alpha.dtype, device=xt_alpha.device))
+            return tuple(out)
+        return tuple(float(theta_g[g]) * st.xtv(alpha, normalize=True).reshape(-1)
+                     for g, st in enumerate(self.streamers))
+
+    def predict(self, effects: EffectEstimates, *, test_fitter: "InfinitesimalREMLFitter",
+                test_covar=None) -> PredictionEstimates:
+        """reml_model.py:1649-1816 (dense branches): X_test beta + sum_g Z_test,g b_g
+        with the test panels standardized by the training statistics."""
+        if not self.streamers or not test_fitter.streamers:
+            raise RuntimeError("Prediction requires initialized training and test streamers.")
+        trains, tests = list(self.streamers), list(test_fitter.streamers)
+        if len(trains) != len(tests):
+            raise ValueError(
+                f"Prediction component count mismatch: expected {len(trains)}, got {len(tests)}.")
+        for idx, (a, b) in enumerate(zip(trains, tests)):
+            if a.m != b.m:
+                raise ValueError(f"Prediction SNP count mismatch for component {idx}: "
+                                 f"expected {a.m}, got {b.m}.")
+            if not (np.array_equal(a._call_snp_starts, b._call_snp_starts)
+                    and np.array_equal(a._call_true_widths, b._call_true_widths)):
+                raise ValueError(f"Prediction call geometry mismatch for component {idx}.")
+            if a.n_components != b.n_components or not np.array_equal(
+                    a._component_snp_offsets, b._component_snp_offsets):
+                raise ValueError(f"Prediction component partition mismatch for component {idx}.")
+        expected = (self._partitioned_streamer.n_components
+                    if self._partitioned_streamer is not None else len(trains))
+        if len(effects.snp_effects) != expected:
+            raise ValueError("Prediction SNP-effect component count mismatch: "
+                             f"expected {expected}, got {len(effects.snp_effects)}.")
+        for a, b in zip(trains, tests):
+            b.copy_standardization_from(a)
+        n_test = int(tests[0].n)
+        dev = torch.device("cuda", torch.cuda.current_device())
+        beta = torch.as_tensor(effects.fixed_effects, dtype=torch.float32, device=dev).reshape(-1)
+        X_test = None
+        if test_covar is not None:
+            X_test = torch.as_tensor(np.asarray(test_covar, np.float32), device=dev)
+            if X_test.ndim == 1:
+                X_test = X_test[:, None]
+            if X_test.ndim != 2 or int(X_test.shape[0]) != n_test:
+                raise ValueError(
+                    "Prediction covariates must be a 2D matrix with one row per test sample.")
+            if not bool(torch.isfinite(X_test).all()):
+                raise ValueError("Prediction covariates contain non-finite values.")
+        if beta.numel() > 0:
+            if X_test is None:
+                raise ValueError(
+                    "Prediction requires test covariates aligned to the training design matrix.")
+            if int(X_test.shape[1]) != int(beta.numel()):
+                raise ValueError(f"Prediction covariate width mismatch: expected "
+                                 f"{int(beta.numel())}, got {int(X_test.shape[1])}.")
+            fixed = X_test @ beta
+        else:
+            fixed = torch.zeros(n_test, dtype=torch.float32, device=dev)
+        if self._partitioned_streamer is not None:
+            st = test_fitter._partitioned_streamer
+            if st is None:
+                raise ValueError("Prediction requires a partitioned test streamer to match training.")
+            b = torch.cat([torch.as_tensor(v, dtype=torch.float32, device=dev).reshape(-1)
+                           for v in effects.snp_effects])
+            stacked = st.panel.zb(b[:, None], 0, st.n_components, stacked=True,
+                                  precision=st._prec)[:, :, 0]
+            comps = tuple(stacked[g] for g in range(st.n_components))
+        else:
+            comps = tuple(
+                st.panel.zb(torch.as_tensor(v, dtype=torch.float32, device=dev).reshape(-1, 1), 0,
+                            st.n_components, precision=st._prec)[:, 0]
+                for st, v in zip(tests, effects.snp_effects))
+        random_effect = sum(comps[1:], comps[0])
+        y_pred_std = fixed + random_effect
+        y_pred = effects.y_mean + effects.y_scale * y_pred_std
+        return PredictionEstimates(fixed_effect=fixed, random_effect=random_effect,
+                                   random_effect_components=comps, y_pred_std=y_pred_std,
+                                   y_pred=y_pred)
+
     def close(self):
         for st in self.streamers:
             st.close()
 
 
-__all__ = ["FitConfig", "FitResult", "InfinitesimalREMLFitter"]
+__all__ = ["EffectEstimates", "FitConfig", "FitResult", "InfinitesimalREMLFitter",
+           "PredictionEstimates"]

@@ -110,6 +110,12 @@ int coh_panel_finalize(coh_panel* panel, const float* dev_mean_override,
  * be NULL). */
 int coh_panel_copy_stats(const coh_panel* panel, float* dev_mean, float* dev_inv,
                          int64_t* dev_cnt, int64_t* dev_s1, int64_t* dev_s2, void* stream);
+/* Replace the float32 mean/inv_sd of a finalized panel (device arrays of m
+ * entries, cache order): a test panel standardized with the training
+ * panel's statistics for prediction (reml_model.py:275-279
+ * `_copy_training_standardization_to_test_streamer`). */
+int coh_panel_set_standardization(coh_panel* panel, const float* dev_mean, const float* dev_inv,
+                                  void* stream);
 /* Read one call back in the reference cache layout (n, ceil(w/4)). */
 int coh_panel_read_packed_call(const coh_panel* panel, int32_t call_idx, uint8_t* dev_out,
                                int64_t ld, void* stream);

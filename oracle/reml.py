@@ -352,6 +352,44 @@ def fit_reml(pb: Problem, y, covar, *, seed=0, h2_init=0.5, maxiter=400, minq_it
     return p
 
 
+def estimate_effects(pb: Problem, Z_by_comp, comp_eff, theta, y, X):
+    """reml_model.py:1544-1647 in FP64 on dense kernels: GLS fixed effects,
+    alpha = H^-1 (y - X beta), random effects theta_g K_g alpha and SNP
+    effects theta_g / eff_g Z_g^T alpha (reml_model.py:1459-1485)."""
+    ys, y_mean, y_scale = standardize_response(np.asarray(y, np.float32))
+    ys = np.asarray(ys, np.float64)
+    theta = np.asarray(theta, np.float32).astype(np.float64)
+    G = pb.G
+    H = pb.H(theta)
+    rhs = ys[:, None] if X is None else np.concatenate([ys[:, None], np.asarray(X, np.float64)], 1)
+    sol = np.linalg.solve(H, rhs)
+    Hinv_y = sol[:, 0]
+    if X is not None:
+        X = np.asarray(X, np.float64)
+        Hinv_x = sol[:, 1:]
+        beta = np.linalg.solve(X.T @ Hinv_x, X.T @ Hinv_y)
+        alpha = Hinv_y - Hinv_x @ beta
+    else:
+        beta = np.zeros(0)
+        alpha = Hinv_y
+    comps = [theta[g] * (pb.Ks[g] @ alpha) for g in range(G)]
+    snp = [theta[g] / float(comp_eff[g]) * (Z_by_comp[g].T @ alpha) if comp_eff[g] > 0
+           else np.zeros(Z_by_comp[g].shape[1]) for g in range(G)]
+    return dict(beta=beta, alpha=alpha, comps=comps, random=np.sum(comps, axis=0), snp=snp,
+                y_mean=float(y_mean), y_scale=float(y_scale), ys=ys)
+
+
+def predict(eff, Z_test_by_comp, X_test):
+    """reml_model.py:1649-1816: X_test beta + sum_g Z_test,g b_g on the
+    standardized scale, then y_mean + y_scale * that."""
+    n = Z_test_by_comp[0].shape[0]
+    fixed = np.zeros(n) if eff["beta"].size == 0 else np.asarray(X_test, np.float64) @ eff["beta"]
+    comps = [Z @ b for Z, b in zip(Z_test_by_comp, eff["snp"])]
+    y_std = fixed + np.sum(comps, axis=0)
+    return dict(fixed=fixed, comps=comps, y_pred_std=y_std,
+                y_pred=eff["y_mean"] + eff["y_scale"] * y_std)
+
+
 def exact_reml_loglik(pb: Problem, p, y, X):
     """Exact dense restricted log-likelihood in the reference's scaling:
     -(y'Py + log|H| + log|X'H^-1X|)/(2n)."""

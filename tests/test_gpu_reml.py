@@ -9,7 +9,7 @@ import torch
 
 from oracle import geno
 from oracle import reml as oreml
-from tests.helpers import int8_block, random_codes
+from tests.helpers import int8_block, random_codes, rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -214,3 +214,54 @@ def test_reorth_primitives_match_fp64_products():
     c1r = torch.bmm(Pd, xd[:, :, None])[:, :, 0]
     w2r = xd - torch.bmm(Pd.transpose(1, 2), c1r[:, :, None])[:, :, 0]
     torch.testing.assert_close(w2.double(), w2r, rtol=1e-5, atol=1e-3)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "i8x2"])
+def test_effects_and_prediction_match_oracle(precision):
+    """estimate_effects / predict (reml_model.py:1544-1816) against the FP64
+    oracle at fixed variance components; test panel standardized with the
+    training statistics."""
+    from coherit_amd.reml_model import FitConfig, InfinitesimalREMLFitter
+    from coherit_amd.sources import ArraySource
+
+    sizes = [500, 700]
+    codes = random_codes(320, 1200, seed=41)
+    codes_test = random_codes(150, 1200, seed=42)
+    kw = dict(vc_block_sizes=sizes, call_width=400, precond_rank=40, seed=3, verbose=False,
+              effect_pcg_tol=1e-6, precision=precision)
+    fitter = InfinitesimalREMLFitter(FitConfig(
+        sources=[ArraySource(int8_block(codes), missing_val=-9)], **kw))
+    test_fitter = InfinitesimalREMLFitter(FitConfig(
+        sources=[ArraySource(int8_block(codes_test), missing_val=-9)], **kw))
+    o = geno.OraclePanel.from_codes(codes, sizes)
+    y, X = _pheno(o, 9)
+    rng = np.random.default_rng(5)
+    X_test = np.concatenate([np.ones((150, 1)), rng.standard_normal((150, 2))], 1).astype(np.float32)
+    theta = np.array([0.25, 0.2, 0.55], np.float32)
+    eff = fitter.estimate_effects(y, var_components=theta, covar=X)
+    pred = fitter.predict(eff, test_fitter=test_fitter, test_covar=X_test)
+
+    off = o.component_offsets
+    Z = o.standardized()
+    Zs = [Z[:, off[g]:off[g + 1]] for g in range(2)]
+    pb = oreml.Problem(_dense(o), o.atoms.component_mean_diag.astype(np.float64))
+    ref = oreml.estimate_effects(pb, Zs, o.atoms.component_eff, theta, y, X)
+    ot = geno.OraclePanel.from_codes(codes_test, sizes, override=(o.mean, o.inv_sd))
+    Zt = ot.standardized()
+    ref_p = oreml.predict(ref, [Zt[:, off[g]:off[g + 1]] for g in range(2)], X_test)
+
+    tol = 1e-4 if precision == "fp32" else 2e-3
+    print(precision, "pcg", eff.pcg_iters, eff.pcg_rel_res, flush=True)
+    np.testing.assert_allclose(eff.fixed_effects.cpu().numpy(), ref["beta"], rtol=tol, atol=tol)
+    assert rel_err(eff.random_effect.cpu().numpy(), ref["random"]) < tol
+    for g in range(2):
+        assert rel_err(eff.random_effect_components[g].cpu().numpy(), ref["comps"][g]) < tol
+        assert rel_err(eff.snp_effects[g].cpu().numpy(), ref["snp"][g]) < tol
+    assert eff.y_mean == pytest.approx(ref["y_mean"], rel=1e-6)
+    assert eff.y_scale == pytest.approx(ref["y_scale"], rel=1e-6)
+    assert rel_err(pred.y_pred_std.cpu().numpy(), ref_p["y_pred_std"]) < tol
+    assert rel_err(pred.y_pred.cpu().numpy(), ref_p["y_pred"]) < tol
+    for g in range(2):
+        assert rel_err(pred.random_effect_components[g].cpu().numpy(), ref_p["comps"][g]) < tol
+    fitter.close()
+    test_fitter.close()

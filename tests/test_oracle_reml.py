@@ -99,3 +99,22 @@ def test_oracle_fit_reaches_exact_reml_optimum_neighbourhood():
     for d in (np.array([0.05, 0]), np.array([0, 0.05]), np.array([-0.05, 0]), np.array([0, -0.05])):
         cand = np.maximum(theta + d * theta, 1e-3)
         assert f(cand) <= best + 5e-3
+
+
+def test_oracle_effects_satisfy_gls_and_blup_identities():
+    # At the GLS/BLUP solution: X^T alpha = 0 and
+    # sum_g theta_g K_g alpha + theta_e alpha = y_s - X beta (H alpha = y_s - X beta).
+    pb, y, X = small_problem(2)
+    rng = np.random.default_rng(8)
+    Zs = [rng.standard_normal((pb.Ks[0].shape[0], 7)) for _ in range(2)]
+    theta = np.array([0.3, 0.2, 0.5])
+    eff = oreml.estimate_effects(pb, Zs, [7.0, 7.0], theta, y, X)
+    np.testing.assert_allclose(X.T @ eff["alpha"], 0.0, atol=1e-9)
+    th = theta.astype(np.float32).astype(np.float64)
+    np.testing.assert_allclose(eff["random"] + th[2] * eff["alpha"], eff["ys"] - X @ eff["beta"],
+                               rtol=1e-9, atol=1e-9)
+    # SNP effects reproduce the random effect of a kernel built from the same Z
+    for g in range(2):
+        np.testing.assert_allclose(eff["snp"][g], th[g] / 7.0 * Zs[g].T @ eff["alpha"], rtol=1e-12)
+    pred = oreml.predict(eff, Zs, X)
+    np.testing.assert_allclose(pred["y_pred"], eff["y_mean"] + eff["y_scale"] * pred["y_pred_std"])
