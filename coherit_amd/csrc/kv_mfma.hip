@@ -103,10 +103,13 @@ __global__ __launch_bounds__(256) void k_colmax_part(int64_t rows, int64_t k,
   if (r < R) {
     for (int64_t i = (int64_t)blockIdx.x * R + r; i < rows; i += (int64_t)gridDim.x * R) {
       const float x = X[i * ldx + c0 + c];
-      mx = fmaxf(mx, fabsf(x));
+      // a non-finite entry counts as +inf (fmaxf would drop a NaN), so its
+      // column's scale becomes NaN and the column's outputs are NaN
+      mx = fmaxf(mx, isfinite(x) ? fabsf(x) : INFINITY);
       if (mean) {
         const float mu = mean[mean_off + i];
-        mx = fmaxf(mx, fabsf(mu * x));
+        const float mux = mu * x;
+        mx = fmaxf(mx, isfinite(mux) ? fabsf(mux) : INFINITY);
         w += static_cast<double>(mu) * static_cast<double>(x);
       }
     }
@@ -128,18 +131,38 @@ __global__ __launch_bounds__(256) void k_colmax_part(int64_t rows, int64_t k,
   }
 }
 
-__global__ void k_colmax_final(const float* __restrict__ part, const double* __restrict__ wpart,
-                               int nblk, int64_t k, double qmax, double* __restrict__ scale,
-                               double* __restrict__ inv_scale, double* __restrict__ wsum) {
-  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < k;
-       c += (int64_t)gridDim.x * blockDim.x) {
-    float m = 0.0f;
-    for (int b = 0; b < nblk; ++b) m = fmaxf(m, part[(int64_t)b * k + c]);
-    if (wsum) {
-      double w = 0.0;
-      for (int b = 0; b < nblk; ++b) w += wpart[(int64_t)b * k + c];
-      wsum[c] = w;
+// One 256-thread block per column: max and FP64 sum over the nblk partials
+// (fixed tree order: deterministic).
+__global__ __launch_bounds__(256) void k_colmax_final(const float* __restrict__ part,
+                                                      const double* __restrict__ wpart, int nblk,
+                                                      int64_t k, double qmax,
+                                                      double* __restrict__ scale,
+                                                      double* __restrict__ inv_scale,
+                                                      double* __restrict__ wsum) {
+  __shared__ float smax[256];
+  __shared__ double ssum[256];
+  const int t = threadIdx.x;
+  for (int64_t c = blockIdx.x; c < k; c += gridDim.x) {
+    float mx = 0.0f;
+    double w = 0.0;
+    for (int b = t; b < nblk; b += 256) {
+      mx = fmaxf(mx, part[(int64_t)b * k + c]);
+      if (wsum) w += wpart[(int64_t)b * k + c];
     }
+    smax[t] = mx;
+    ssum[t] = w;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+      if (t < off) {
+        smax[t] = fmaxf(smax[t], smax[t + off]);
+        ssum[t] += ssum[t + off];
+      }
+      __syncthreads();
+    }
+    const float m = smax[0];
+    if (t == 0 && wsum) wsum[c] = ssum[0];
+    __syncthreads();
+    if (t != 0) continue;
     // a non-finite column propagates as NaN (its limbs are zero)
     const bool finite = isfinite(m);
     const double s = (m > 0.0f && finite) ? static_cast<double>(m) / qmax : 1.0;
@@ -243,6 +266,36 @@ __global__ __launch_bounds__(256) void k_quant_p2(const float* __restrict__ U, i
   }
 }
 
+// --------------------------------------------------------- workgroup map
+//
+// Workgroups are dispatched to the 8 XCDs round-robin by linear id, each XCD
+// with its own L2.  The column groups of one (tile, split) unit stream the
+// same genotype rows, so they are given ids 8 apart: same XCD, adjacent in
+// its queue, and the second group's genotype reads hit that XCD's L2.
+constexpr int kXcds = 8;
+
+struct WgMap {
+  int64_t bx, by;
+  int bz;
+  bool valid;
+};
+
+__device__ inline WgMap xcd_map(int64_t ux, int64_t uy, int groups) {
+  const int64_t w = blockIdx.x;
+  const int64_t q = w / kXcds;
+  const int64_t u = (q / groups) * kXcds + w % kXcds;
+  WgMap m;
+  m.bz = static_cast<int>(q % groups);
+  m.valid = u < ux * uy;
+  m.bx = u % ux;
+  m.by = u / ux;
+  return m;
+}
+
+static unsigned xcd_grid(int64_t ux, int64_t uy, int64_t groups) {
+  return (unsigned)(ceil_div(ux * uy, kXcds) * kXcds * groups);
+}
+
 // --------------------------------------------------------------- phase 1
 
 struct P1Args {
@@ -263,6 +316,8 @@ struct P1Args {
   float* U;            // (nslots, k) when nsplit == 1
   int32_t* part;       // [split][nslots][k][2L] when nsplit > 1
   int nsplit;
+  int64_t ux, uy;      // slot tiles x sample splits
+  int groups;          // column groups
 };
 
 template <int L, int NB, int MB, int T, int PD>
@@ -276,10 +331,12 @@ __global__ __launch_bounds__(256) void k_p1_mfma(P1Args a) {
   __shared__ v4i sB[2][STG];
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
   const int r = lane % 16, g = lane / 16;
-  const int64_t tile0 = (int64_t)blockIdx.x * (64 * MB) + wave * (16 * MB);
-  const int nbg0 = blockIdx.z * NB;  // first 16-col block of this column group
+  const WgMap wg = xcd_map(a.ux, a.uy, a.groups);
+  if (!wg.valid) return;
+  const int64_t tile0 = wg.bx * (64 * MB) + wave * (16 * MB);
+  const int nbg0 = wg.bz * NB;  // first 16-col block of this column group
   const int64_t nsb = a.KB / 4;
-  const int64_t sb0 = (int64_t)blockIdx.y * a.sb_per_split;
+  const int64_t sb0 = wg.by * a.sb_per_split;
   const int64_t sb1 = min(nsb, sb0 + a.sb_per_split);
   const int64_t ks_end = sb1 * 4;
   v4i accg[L][MB][NB], acco[L][MB][NB];
@@ -407,7 +464,7 @@ __global__ __launch_bounds__(256) void k_p1_mfma(P1Args a) {
           const double tv = a.scale[col] * (A1 - static_cast<double>(a.mean[slot]) * A2);
           a.U[jl * a.k + col] = static_cast<float>(f * tv);
         } else {
-          int32_t* dst = a.part + (((int64_t)blockIdx.y * a.nslots + jl) * a.k + col) * (2 * L);
+          int32_t* dst = a.part + ((wg.by * a.nslots + jl) * a.k + col) * (2 * L);
 #pragma unroll
           for (int l = 0; l < L; ++l) {
             dst[2 * l] = accg[l][mb][nb][v];
@@ -462,6 +519,8 @@ struct P2Args {
   const double* scale;
   ChunkTable tab;
   int32_t* part;       // [chunk][n][k][L]
+  int64_t ux, uy;      // sample tiles x slot chunks
+  int groups;          // column groups
 };
 
 template <int L, int NB, int MB, int T, int PD>
@@ -472,10 +531,12 @@ __global__ __launch_bounds__(256) void k_p2_mfma(P2Args a) {
   __shared__ v4i sB[2][STG];
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
   const int r = lane % 16, g = lane / 16;
-  const int64_t i0 = (int64_t)blockIdx.x * (64 * MB) + wave * (16 * MB);
-  const int nbg0 = blockIdx.z * NB;
+  const WgMap wg = xcd_map(a.ux, a.uy, a.groups);
+  if (!wg.valid) return;
+  const int64_t i0 = wg.bx * (64 * MB) + wave * (16 * MB);
+  const int nbg0 = wg.bz * NB;
   int64_t j0, j1;
-  a.tab.chunk_range(blockIdx.y, &j0, &j1);
+  a.tab.chunk_range(static_cast<int>(wg.by), &j0, &j1);
   const int64_t sb0 = (j0 - a.slot0) / 256, sb1 = (j1 - a.slot0) / 256;
   const int64_t ks_end = sb1 * 4;
   v4i acc[L][MB][NB];
@@ -590,7 +651,7 @@ __global__ __launch_bounds__(256) void k_p2_mfma(P2Args a) {
       for (int v = 0; v < 4; ++v) {
         const int64_t i = i0 + mb * 16 + 4 * g + v;
         if (i >= a.n) continue;
-        int32_t* dst = a.part + (((int64_t)blockIdx.y * a.n + i) * a.k + col) * L;
+        int32_t* dst = a.part + ((wg.by * a.n + i) * a.k + col) * L;
 #pragma unroll
         for (int l = 0; l < L; ++l) dst[l] = acc[l][mb][nb][v];
       }
@@ -763,8 +824,8 @@ static int colscale(const float* X, int64_t rows, int64_t k, int64_t ldx, const 
   hipLaunchKernelGGL(k_colmax_part, dim3(nblk, (unsigned)ceil_div(k, 256)), dim3(256), 0, st, rows,
                      k, X, ldx, mean, mean_off, part, wpart);
   COH_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_colmax_final, dim3((unsigned)ceil_div(k, 256)), dim3(256), 0, st, part,
-                     wpart, nblk, k, qmax, scale, inv_scale, mean ? wsum : nullptr);
+  hipLaunchKernelGGL(k_colmax_final, dim3((unsigned)std::min<int64_t>(k, 1024)), dim3(256), 0, st,
+                     part, wpart, nblk, k, qmax, scale, inv_scale, mean ? wsum : nullptr);
   COH_CHECK_LAUNCH();
   return COH_OK;
 }
@@ -806,7 +867,10 @@ static int phase1(const coh_panel* p, const KvPlan& pl, const float* V, int64_t 
   a.U = reinterpret_cast<float*>(ws + pl.off_U);
   a.part = reinterpret_cast<int32_t*>(ws + pl.off_p1);
   a.nsplit = pl.nsplit1;
-  dim3 grid((unsigned)(nslots / (64 * MB)), (unsigned)pl.nsplit1, (unsigned)groups);
+  a.ux = nslots / (64 * MB);
+  a.uy = pl.nsplit1;
+  a.groups = (int)groups;
+  dim3 grid(xcd_grid(a.ux, a.uy, groups));
   char name[64];
   snprintf(name, sizeof(name), "k_p1_mfma_L%d_NB%d_MB%d_T%d_PD%d", L, NB, MB, T, PD);
   KernelTimer tm(st, name, (double)p->n * (double)ceil_div(pl.m_sel, 4),
@@ -876,7 +940,10 @@ static int phase2(const coh_panel* p, const KvPlan& pl, int64_t k, int stacked, 
   a.scale = scale;
   a.tab = pl.tab;
   a.part = reinterpret_cast<int32_t*>(ws + pl.off_p1);
-  dim3 grid((unsigned)ceil_div(p->n, 64 * MB), (unsigned)pl.tab.nchunks, (unsigned)groups);
+  a.ux = ceil_div(p->n, 64 * MB);
+  a.uy = pl.tab.nchunks;
+  a.groups = (int)groups;
+  dim3 grid(xcd_grid(a.ux, a.uy, groups));
   char name[64];
   snprintf(name, sizeof(name), "k_p2_mfma_L%d_NB%d_MB%d_T%d_PD%d", L, NB, MB, T, PD);
   KernelTimer tm(st, name, (double)p->n * (double)ceil_div(pl.m_sel, 4),

@@ -247,3 +247,23 @@ def test_every_compiled_mfma_shape_gives_identical_results(partitioned, monkeypa
         np.testing.assert_array_equal(got, ref)
     monkeypatch.delenv("COH_MFMA_SHAPE")
     st.set_precision("fp32")
+
+
+@pytest.mark.parametrize("precision", ["fp32", "i8x3", "i8x2"])
+def test_non_finite_input_column_propagates(partitioned, precision):
+    """A NaN or inf entry of V makes its output column non-finite (as FP32
+    arithmetic does), so PCG/SLQ breakdown checks see it; other columns are
+    unaffected."""
+    st, o = partitioned
+    st.set_precision(precision)
+    rng = np.random.default_rng(3)
+    V = rng.standard_normal((o.n, 5)).astype(np.float32)
+    V[17, 1] = np.nan
+    V[3, 3] = np.inf
+    out = st.kv(torch.as_tensor(V, device="cuda")).cpu().numpy()
+    assert not np.isfinite(out[:, 1]).any()
+    assert not np.isfinite(out[:, 3]).any()
+    good = [0, 2, 4]
+    assert np.isfinite(out[:, good]).all()
+    assert rel_err(out[:, good], okv.kv(o, V[:, good])) < TOL[precision]
+    st.set_precision("fp32")
