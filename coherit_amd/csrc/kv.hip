@@ -9,14 +9,10 @@
 // inv * scale for X^T V.  Both phases split their contraction axis over
 // blocks and reduce the partials in a fixed order (deterministic results).
 //
-// Precision COH_PREC_FP32 runs on the VALU: the centred value is formed in
-// FP32 exactly as the reference does and multiplied in FP32; FMAs accumulate
-// in FP32 over runs of kRun terms that are flushed into FP64 accumulators, and
-// split partials are reduced in FP64.  A plain FP32 running sum over ~1e5
-// samples loses ~1e-2 absolute on sums of centred codes whose exact value is
-// ~1e-3 (the intercept column of X^T V), which puts a ~1e-4 floor under PCG
-// residuals at n = 1e5; the blocked sum removes it at ~3% extra VALU work.
-// The int8-limb MFMA precisions live in kv_mfma.hip.
+// This file holds the shared pieces: FP64 split-partial reductions, row
+// gathers, the effect scatter of Z b and the extern "C" operator entry
+// points.  The contraction kernels live in kv_f32.hip (plain FP32 on the
+// matrix cores) and kv_mfma.hip (int8 limbs).
 #include "common.hpp"
 #include "kv_plan.hpp"
 #include "timing.hpp"
@@ -26,147 +22,7 @@
 
 namespace coh {
 
-// ------------------------------------------------------------ VALU kernels
-
-constexpr int kRun = 32;  // FP32 accumulation run before an FP64 flush
-
-template <int KC>
-__device__ inline void flush_run(double tot[4][KC], float acc[4][KC]) {
-#pragma unroll
-  for (int s = 0; s < 4; ++s)
-#pragma unroll
-    for (int c = 0; c < KC; ++c) {
-      tot[s][c] += static_cast<double>(acc[s][c]);
-      acc[s][c] = 0.0f;
-    }
-}
-
-template <int KC>
-__device__ inline void p1_accum(uint32_t b, const float* __restrict__ vrow, int kvalid,
-                                const float mu[4], float acc[4][KC]) {
-  float v[KC];
-#pragma unroll
-  for (int c = 0; c < KC; ++c) v[c] = (c < kvalid) ? vrow[c] : 0.0f;
-#pragma unroll
-  for (int s = 0; s < 4; ++s) {
-    const uint32_t code = (b >> (2 * s)) & 3u;
-    const float d = (code == 3u) ? 0.0f : (static_cast<float>(code) - mu[s]);
-#pragma unroll
-    for (int c = 0; c < KC; ++c) acc[s][c] = fmaf(d, v[c], acc[s][c]);
-  }
-}
-
-// One thread per panel byte column (4 slots); loops over a sample range.
-template <int KC>
-__global__ __launch_bounds__(256) void k_p1_valu(const uint8_t* __restrict__ smaj, int64_t sbytes,
-                                                 int64_t n, int64_t byte0, int64_t nbytes,
-                                                 const float* __restrict__ mean,
-                                                 const float* __restrict__ V, int64_t ldv,
-                                                 int kvalid, int64_t rows_per_split,
-                                                 double* __restrict__ part) {
-  const int64_t qb = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  const bool act = qb < nbytes;
-  const int64_t q = byte0 + (act ? qb : 0);
-  const int64_t i0 = (int64_t)blockIdx.y * rows_per_split;
-  const int64_t i1 = min(n, i0 + rows_per_split);
-  float mu[4];
-#pragma unroll
-  for (int s = 0; s < 4; ++s) mu[s] = mean[4 * q + s];
-  float acc[4][KC];
-  double tot[4][KC];
-#pragma unroll
-  for (int s = 0; s < 4; ++s)
-#pragma unroll
-    for (int c = 0; c < KC; ++c) {
-      acc[s][c] = 0.0f;
-      tot[s][c] = 0.0;
-    }
-  const uint8_t* col = smaj + q;
-  int64_t i = i0;
-  while (i < i1) {
-    const int64_t iend = min(i1, i + kRun);
-    for (; i + 4 <= iend; i += 4) {
-      const uint32_t b0 = col[(i + 0) * sbytes];
-      const uint32_t b1 = col[(i + 1) * sbytes];
-      const uint32_t b2 = col[(i + 2) * sbytes];
-      const uint32_t b3 = col[(i + 3) * sbytes];
-      p1_accum<KC>(b0, V + (i + 0) * ldv, kvalid, mu, acc);
-      p1_accum<KC>(b1, V + (i + 1) * ldv, kvalid, mu, acc);
-      p1_accum<KC>(b2, V + (i + 2) * ldv, kvalid, mu, acc);
-      p1_accum<KC>(b3, V + (i + 3) * ldv, kvalid, mu, acc);
-    }
-    for (; i < iend; ++i) p1_accum<KC>(col[i * sbytes], V + i * ldv, kvalid, mu, acc);
-    flush_run<KC>(tot, acc);
-  }
-  if (!act) return;
-  double* out = part + ((int64_t)blockIdx.y * nbytes + qb) * 4 * KC;
-#pragma unroll
-  for (int s = 0; s < 4; ++s)
-#pragma unroll
-    for (int c = 0; c < KC; ++c) out[s * KC + c] = tot[s][c];
-}
-
-template <int KC>
-__device__ inline void p2_accum(uint32_t byte, float mu, const float* __restrict__ urow,
-                                int kvalid, float acc[4][KC]) {
-  float u[KC];
-#pragma unroll
-  for (int c = 0; c < KC; ++c) u[c] = (c < kvalid) ? urow[c] : 0.0f;
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const uint32_t code = (byte >> (2 * r)) & 3u;
-    const float d = (code == 3u) ? 0.0f : (static_cast<float>(code) - mu);
-#pragma unroll
-    for (int c = 0; c < KC; ++c) acc[r][c] = fmaf(d, u[c], acc[r][c]);
-  }
-}
-
-// One thread per variant-major byte (4 samples); loops over a slot chunk.
-template <int KC>
-__global__ __launch_bounds__(256) void k_p2_valu(const uint8_t* __restrict__ vmaj, int64_t vbytes,
-                                                 int64_t n, const float* __restrict__ mean,
-                                                 const float* __restrict__ U, int64_t ldu,
-                                                 int64_t slot_base, int kvalid, ChunkTable tab,
-                                                 double* __restrict__ part) {
-  const int64_t nb = (n + 3) / 4;
-  const int64_t b = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  const bool act = b < nb;
-  int64_t j0, j1;
-  tab.chunk_range(blockIdx.y, &j0, &j1);
-  float acc[4][KC];
-  double tot[4][KC];
-#pragma unroll
-  for (int r = 0; r < 4; ++r)
-#pragma unroll
-    for (int c = 0; c < KC; ++c) {
-      acc[r][c] = 0.0f;
-      tot[r][c] = 0.0;
-    }
-  const uint8_t* col = vmaj + (act ? b : 0);
-  int64_t j = j0;
-  while (j < j1) {
-    const int64_t jend = min(j1, j + kRun);
-    for (; j + 4 <= jend; j += 4) {
-      const uint32_t y0 = col[(j + 0) * vbytes];
-      const uint32_t y1 = col[(j + 1) * vbytes];
-      const uint32_t y2 = col[(j + 2) * vbytes];
-      const uint32_t y3 = col[(j + 3) * vbytes];
-      p2_accum<KC>(y0, mean[j + 0], U + (j + 0 - slot_base) * ldu, kvalid, acc);
-      p2_accum<KC>(y1, mean[j + 1], U + (j + 1 - slot_base) * ldu, kvalid, acc);
-      p2_accum<KC>(y2, mean[j + 2], U + (j + 2 - slot_base) * ldu, kvalid, acc);
-      p2_accum<KC>(y3, mean[j + 3], U + (j + 3 - slot_base) * ldu, kvalid, acc);
-    }
-    for (; j < jend; ++j)
-      p2_accum<KC>(col[j * vbytes], mean[j], U + (j - slot_base) * ldu, kvalid, acc);
-    flush_run<KC>(tot, acc);
-  }
-  if (!act) return;
-  double* out = part + (int64_t)blockIdx.y * nb * 4 * KC + b * 4 * KC;
-#pragma unroll
-  for (int r = 0; r < 4; ++r)
-#pragma unroll
-    for (int c = 0; c < KC; ++c) out[r * KC + c] = tot[r][c];
-}
+// ----------------------------------------------------------- reductions
 
 // U[local, c0+c] = factor(slot) * sum_split part[split][local][c]
 __global__ void k_reduce_p1(const double* __restrict__ part, int nsplit, int64_t nslots, int KC,
@@ -256,109 +112,21 @@ static unsigned grid_for(int64_t total) {
 
 // ------------------------------------------------------------- planning
 
-static int kc_for(int64_t k) {
-  if (k <= 1) return 1;
-  if (k <= 2) return 2;
-  if (k <= 4) return 4;
-  if (k <= 8) return 8;
-  return 16;
-}
-
-int plan_valu(const coh_panel* p, int64_t k, int32_t cb, int32_t ce, KvPlan* pl) {
-  pl->slot0 = p->comp_slot_begin[cb];
-  pl->slot1 = p->comp_slot_end[ce - 1];
-  const int64_t nslots = pl->slot1 - pl->slot0;
-  pl->KC = kc_for(k);
-  const int64_t nbytes1 = nslots / 4;
-  const int64_t xb1 = std::max<int64_t>(1, ceil_div(nbytes1, 256));
-  int64_t nsplit = std::max<int64_t>(1, ceil_div(kTargetBlocks, xb1));
-  nsplit = std::min<int64_t>(nsplit, std::max<int64_t>(1, p->n / 64));
-  nsplit = std::min<int64_t>(nsplit, 256);
-  pl->nsplit1 = static_cast<int>(nsplit);
-  pl->rows_per_split = ceil_div(p->n, nsplit);
-  const int64_t nb = ceil_div(p->n, 4);
-  const int64_t xb2 = std::max<int64_t>(1, ceil_div(nb, 256));
-  const int64_t target_chunks = std::max<int64_t>(1, ceil_div(kTargetBlocks, xb2));
-  pl->chunk_len = std::max<int64_t>(256, round_up(ceil_div(nslots, target_chunks), 256));
-  build_chunks(p, cb, ce, pl->chunk_len, &pl->tab);
-  pl->off_U = 0;
-  pl->ldu = k;
-  int64_t off = round_up(nslots * k * 4, 256);
-  pl->off_p1 = off;
-  off += round_up((int64_t)pl->nsplit1 * nslots * pl->KC * 8, 256);
-  pl->off_p2 = off;
-  off += round_up((int64_t)pl->tab.nchunks * nb * 4 * pl->KC * 8, 256);
-  pl->total = off;
+int reduce_p1_partials(const KvPlan& pl, const double* part, int nsplit, int64_t nslots, int64_t k,
+                       const float* inv, const CompScale& cs, int inv_pow, float* U,
+                       hipStream_t st) {
+  hipLaunchKernelGGL(k_reduce_p1, dim3(grid_for(nslots * k)), dim3(256), 0, st, part, nsplit,
+                     nslots, (int)k, (int)k, 0, inv, pl.slot0, cs, inv_pow, U, pl.ldu);
+  COH_CHECK_LAUNCH();
   return COH_OK;
 }
 
-int run_phase1_valu(const coh_panel* p, const KvPlan& pl, const float* V, int64_t k, int64_t ldv,
-                    const CompScale& cs, int inv_pow, uint8_t* ws, hipStream_t st) {
-  const int64_t nslots = pl.slot1 - pl.slot0;
-  const int64_t nbytes1 = nslots / 4;
-  float* U = reinterpret_cast<float*>(ws + pl.off_U);
-  double* part = reinterpret_cast<double*>(ws + pl.off_p1);
-  dim3 grid((unsigned)ceil_div(nbytes1, 256), (unsigned)pl.nsplit1);
-  for (int64_t c0 = 0; c0 < k; c0 += pl.KC) {
-    const int kvalid = static_cast<int>(std::min<int64_t>(pl.KC, k - c0));
-    const float* Vc = V + c0;
-#define COH_P1(KCV)                                                                          \
-  hipLaunchKernelGGL(k_p1_valu<KCV>, grid, dim3(256), 0, st, p->smaj, p->sbytes, p->n,        \
-                     pl.slot0 / 4, nbytes1, p->mean, Vc, ldv, kvalid, pl.rows_per_split, part)
-    char name1[64];
-    snprintf(name1, sizeof(name1), "k_p1_valu_KC%d", pl.KC);
-    const double fl1 = 2.0 * (double)p->n * (double)pl.m_sel * (double)kvalid;
-    KernelTimer tm1(st, name1, (double)p->n * (double)ceil_div(pl.m_sel, 4), fl1, fl1);
-    switch (pl.KC) {
-      case 1: COH_P1(1); break;
-      case 2: COH_P1(2); break;
-      case 4: COH_P1(4); break;
-      case 8: COH_P1(8); break;
-      default: COH_P1(16); break;
-    }
-    tm1.stop();
-#undef COH_P1
-    COH_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_reduce_p1, dim3(grid_for(nslots * pl.KC)), dim3(256), 0, st, part,
-                       pl.nsplit1, nslots, pl.KC, kvalid, (int)c0, p->inv, pl.slot0, cs, inv_pow,
-                       U, pl.ldu);
-    COH_CHECK_LAUNCH();
-  }
-  return COH_OK;
-}
-
-int run_phase2_valu(const coh_panel* p, const KvPlan& pl, int64_t k, int stacked, float* out,
-                    int64_t ldo, int64_t out_stride, float beta, uint8_t* ws, hipStream_t st) {
-  const float* U = reinterpret_cast<const float*>(ws + pl.off_U);
-  double* part = reinterpret_cast<double*>(ws + pl.off_p2);
-  const int64_t nb = ceil_div(p->n, 4);
-  dim3 grid((unsigned)ceil_div(nb, 256), (unsigned)pl.tab.nchunks);
+int reduce_p2_partials(const KvPlan& pl, const double* part, int64_t n, int64_t k, int stacked,
+                       float* out, int64_t ldo, int64_t out_stride, float beta, hipStream_t st) {
   const int nout = stacked ? pl.tab.ncomp : 1;
-  for (int64_t c0 = 0; c0 < k; c0 += pl.KC) {
-    const int kvalid = static_cast<int>(std::min<int64_t>(pl.KC, k - c0));
-    const float* Uc = U + c0;
-#define COH_P2(KCV)                                                                          \
-  hipLaunchKernelGGL(k_p2_valu<KCV>, grid, dim3(256), 0, st, p->vmaj, p->vbytes, p->n, p->mean, \
-                     Uc, pl.ldu, pl.slot0, kvalid, pl.tab, part)
-    char name2[64];
-    snprintf(name2, sizeof(name2), "k_p2_valu_KC%d", pl.KC);
-    const double fl2 = 2.0 * (double)p->n * (double)pl.m_sel * (double)kvalid;
-    KernelTimer tm2(st, name2, (double)p->n * (double)ceil_div(pl.m_sel, 4), fl2, fl2);
-    switch (pl.KC) {
-      case 1: COH_P2(1); break;
-      case 2: COH_P2(2); break;
-      case 4: COH_P2(4); break;
-      case 8: COH_P2(8); break;
-      default: COH_P2(16); break;
-    }
-    tm2.stop();
-#undef COH_P2
-    COH_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_reduce_p2, dim3(grid_for((int64_t)nout * p->n * pl.KC)), dim3(256), 0, st,
-                       part, p->n, pl.KC, kvalid, (int)c0, pl.tab, nout, stacked, out, ldo,
-                       out_stride, beta);
-    COH_CHECK_LAUNCH();
-  }
+  hipLaunchKernelGGL(k_reduce_p2, dim3(grid_for((int64_t)nout * n * k)), dim3(256), 0, st, part, n,
+                     (int)k, (int)k, 0, pl.tab, nout, stacked, out, ldo, out_stride, beta);
+  COH_CHECK_LAUNCH();
   return COH_OK;
 }
 
@@ -509,6 +277,6 @@ extern "C" int coh_zb(const coh_panel* p, int32_t precision, const float* B, int
                      pl.ldu);
   COH_CHECK_LAUNCH();
   if (precision == COH_PREC_FP32)
-    return run_phase2_valu(p, pl, k, stacked, out, ldo, out_stride, beta, w, s);
+    return run_phase2_f32(p, pl, k, stacked, out, ldo, out_stride, beta, w, s);
   return run_phase2_mfma(p, pl, k, stacked, out, ldo, out_stride, beta, w, s);
 }

@@ -40,7 +40,7 @@ struct ChunkTable {
 struct KvPlan {
   int precision;
   int64_t slot0, slot1;     // slot range of the requested components
-  int KC;                   // VALU column chunk
+  int KC;                   // columns per column group
   int nsplit1;              // phase-1 sample splits
   int64_t rows_per_split;
   int64_t chunk_len;
@@ -63,11 +63,20 @@ inline int64_t selected_snps(const coh_panel* p, int32_t cb, int32_t ce) {
 }
 
 void build_chunks(const coh_panel* p, int32_t cb, int32_t ce, int64_t chunk_len, ChunkTable* t);
-int plan_valu(const coh_panel* p, int64_t k, int32_t cb, int32_t ce, KvPlan* pl);
-int run_phase1_valu(const coh_panel* p, const KvPlan& pl, const float* V, int64_t k, int64_t ldv,
-                    const CompScale& cs, int inv_pow, uint8_t* ws, hipStream_t st);
-int run_phase2_valu(const coh_panel* p, const KvPlan& pl, int64_t k, int stacked, float* out,
-                    int64_t ldo, int64_t out_stride, float beta, uint8_t* ws, hipStream_t st);
+
+// FP32 split-partial reductions (kv.hip), shared by the FP32 kernels
+int reduce_p1_partials(const KvPlan& pl, const double* part, int nsplit, int64_t nslots, int64_t k,
+                       const float* inv, const CompScale& cs, int inv_pow, float* U,
+                       hipStream_t st);
+int reduce_p2_partials(const KvPlan& pl, const double* part, int64_t n, int64_t k, int stacked,
+                       float* out, int64_t ldo, int64_t out_stride, float beta, hipStream_t st);
+
+// plain FP32 on the matrix cores (kv_f32.hip)
+int plan_f32(const coh_panel* p, int64_t k, int32_t cb, int32_t ce, KvPlan* pl);
+int run_phase1_f32(const coh_panel* p, const KvPlan& pl, const float* V, int64_t k, int64_t ldv,
+                   const CompScale& cs, int inv_pow, uint8_t* ws, hipStream_t st);
+int run_phase2_f32(const coh_panel* p, const KvPlan& pl, int64_t k, int stacked, float* out,
+                   int64_t ldo, int64_t out_stride, float beta, uint8_t* ws, hipStream_t st);
 
 // MFMA int8-limb path (kv_mfma.hip)
 int plan_mfma(const coh_panel* p, int precision, int64_t k, int32_t cb, int32_t ce, KvPlan* pl);
@@ -80,7 +89,7 @@ inline int make_plan(const coh_panel* p, int precision, int64_t k, int32_t cb, i
                      KvPlan* pl) {
   pl->precision = precision;
   pl->m_sel = selected_snps(p, cb, ce);
-  if (precision == COH_PREC_FP32) return plan_valu(p, k, cb, ce, pl);
+  if (precision == COH_PREC_FP32) return plan_f32(p, k, cb, ce, pl);
   if (precision == COH_PREC_I8X3 || precision == COH_PREC_I8X2)
     return plan_mfma(p, precision, k, cb, ce, pl);
   return fail(COH_ERR_VALUE, "unknown precision %d", precision);
@@ -88,7 +97,7 @@ inline int make_plan(const coh_panel* p, int precision, int64_t k, int32_t cb, i
 
 inline int run_xtv(const coh_panel* p, const KvPlan& pl, int precision, const float* V,
                    int64_t k, int64_t ldv, const CompScale& cs, uint8_t* ws, hipStream_t st) {
-  if (precision == COH_PREC_FP32) return run_phase1_valu(p, pl, V, k, ldv, cs, 1, ws, st);
+  if (precision == COH_PREC_FP32) return run_phase1_f32(p, pl, V, k, ldv, cs, 1, ws, st);
   return run_phase1_mfma(p, pl, V, k, ldv, cs, 1, ws, st);
 }
 
@@ -97,9 +106,9 @@ inline int run_kv(const coh_panel* p, const KvPlan& pl, int precision, const flo
                   int64_t out_stride, float beta, uint8_t* ws, hipStream_t st) {
   int s;
   if (precision == COH_PREC_FP32) {
-    s = run_phase1_valu(p, pl, V, k, ldv, cs, 2, ws, st);
+    s = run_phase1_f32(p, pl, V, k, ldv, cs, 2, ws, st);
     if (s != COH_OK) return s;
-    return run_phase2_valu(p, pl, k, stacked, out, ldo, out_stride, beta, ws, st);
+    return run_phase2_f32(p, pl, k, stacked, out, ldo, out_stride, beta, ws, st);
   }
   s = run_phase1_mfma(p, pl, V, k, ldv, cs, 2, ws, st);
   if (s != COH_OK) return s;
