@@ -200,8 +200,46 @@ extern "C" int coh_kv(const coh_panel* p, int32_t precision, const float* V, int
     cs.scale[g] = comp_scale[cb + g];
   }
   hipStream_t s = (hipStream_t)stream;
-  return run_kv(p, pl, precision, V, k, ldv, cs, stacked, out, ldo, out_stride, beta,
-                static_cast<uint8_t*>(ws), s);
+  return run_kv(p, pl, precision, V, k, ldv, cs, panel_stats(p), stacked, out, ldo, out_stride,
+                beta, static_cast<uint8_t*>(ws), s);
+}
+
+extern "C" int coh_kv_sets(const coh_panel* p, int32_t precision, const float* V, int64_t k,
+                           int64_t ldv, int32_t cb, int32_t ce, int32_t nsets,
+                           const float* set_mean, const float* set_inv, int64_t set_stride,
+                           const int32_t* col_set, const float* set_comp_scale, int32_t stacked,
+                           float* out, int64_t ldo, int64_t out_stride, float beta, void* ws,
+                           uint64_t ws_bytes, void* stream) {
+  int st = validate_common(p, V, k, ldv, cb, ce);
+  if (st != COH_OK) return st;
+  COH_REQUIRE(precision == COH_PREC_I8X3 || precision == COH_PREC_I8X2, COH_ERR_VALUE,
+              "statistic sets need an int8-limb precision (the plain FP32 path takes one set "
+              "through coh_panel_set_standardization)");
+  COH_REQUIRE(nsets >= 1, COH_ERR_VALUE, "nsets must be positive");
+  COH_REQUIRE(set_mean && set_inv && col_set && set_comp_scale, COH_ERR_VALUE,
+              "NULL statistic-set argument");
+  COH_REQUIRE(set_stride >= p->m_slot, COH_ERR_VALUE, "set_stride %lld < m_slot %lld",
+              (long long)set_stride, (long long)p->m_slot);
+  COH_REQUIRE(out != nullptr, COH_ERR_VALUE, "NULL out");
+  COH_REQUIRE(ldo >= k, COH_ERR_VALUE, "ldo < k");
+  COH_REQUIRE(beta == 0.0f || beta == 1.0f, COH_ERR_VALUE, "beta must be 0 or 1");
+  COH_REQUIRE(!stacked || out_stride >= p->n * ldo || ce - cb == 1, COH_ERR_VALUE,
+              "stacked outputs overlap (out_stride too small)");
+  KvPlan pl{};
+  st = make_plan(p, precision, k, cb, ce, &pl);
+  if (st != COH_OK) return st;
+  COH_REQUIRE(ws != nullptr && ws_bytes >= (uint64_t)pl.total, COH_ERR_VALUE,
+              "workspace too small: %llu < %lld bytes", (unsigned long long)ws_bytes,
+              (long long)pl.total);
+  CompScale cs{};
+  cs.ncomp = ce - cb;
+  for (int g = 0; g < cs.ncomp; ++g) {
+    cs.end[g] = p->comp_slot_end[cb + g];
+    cs.scale[g] = 0.0f;  // the per-set factors apply
+  }
+  const StatSets ss{set_mean, set_inv, set_stride, nsets, col_set, set_comp_scale};
+  return run_kv(p, pl, precision, V, k, ldv, cs, ss, stacked, out, ldo, out_stride, beta,
+                static_cast<uint8_t*>(ws), (hipStream_t)stream);
 }
 
 extern "C" int coh_xtv(const coh_panel* p, int32_t precision, const float* V, int64_t k,
@@ -278,5 +316,5 @@ extern "C" int coh_zb(const coh_panel* p, int32_t precision, const float* B, int
   COH_CHECK_LAUNCH();
   if (precision == COH_PREC_FP32)
     return run_phase2_f32(p, pl, k, stacked, out, ldo, out_stride, beta, w, s);
-  return run_phase2_mfma(p, pl, k, stacked, out, ldo, out_stride, beta, w, s);
+  return run_phase2_mfma(p, pl, k, panel_stats(p), stacked, out, ldo, out_stride, beta, w, s);
 }

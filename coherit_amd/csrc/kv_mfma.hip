@@ -86,12 +86,14 @@ __device__ inline int quant(float x, double inv_scale, double qmax) {
 
 // ---------------------------------------------------------- column scales
 
-// part[blk][c] = max_i |X[i,c]| (and |mean_i X[i,c]| when mean != null)
-// Per-block column max |x| (and |mean*x|); with mean, also the FP64 partial
-// column sums of mean*x (fixed thread and block order: deterministic).
+// part[blk][c] = max_i |X[i,c]| (and |mean_i X[i,c]| when with_mean)
+// Per-block column max |x| (and |mean*x|); with the mean, also the FP64
+// partial column sums of mean*x (fixed thread and block order:
+// deterministic).  Column c takes its mean from its statistic set, rows
+// mean_off + i.
 __global__ __launch_bounds__(256) void k_colmax_part(int64_t rows, int64_t k,
                                                      const float* __restrict__ X, int64_t ldx,
-                                                     const float* __restrict__ mean,
+                                                     int with_mean, StatSets ss,
                                                      int64_t mean_off, float* __restrict__ part,
                                                      double* __restrict__ wpart) {
   const int64_t c0 = (int64_t)blockIdx.y * 256;
@@ -100,6 +102,7 @@ __global__ __launch_bounds__(256) void k_colmax_part(int64_t rows, int64_t k,
   const int t = threadIdx.x, c = t % kk, r = t / kk;
   float mx = 0.0f;
   double w = 0.0;
+  const float* mean = with_mean ? ss.mean + ss.set_of(c0 + c) * ss.stride + mean_off : nullptr;
   if (r < R) {
     for (int64_t i = (int64_t)blockIdx.x * R + r; i < rows; i += (int64_t)gridDim.x * R) {
       const float x = X[i * ldx + c0 + c];
@@ -107,7 +110,7 @@ __global__ __launch_bounds__(256) void k_colmax_part(int64_t rows, int64_t k,
       // column's scale becomes NaN and the column's outputs are NaN
       mx = fmaxf(mx, isfinite(x) ? fabsf(x) : INFINITY);
       if (mean) {
-        const float mu = mean[mean_off + i];
+        const float mu = mean[i];
         const float mux = mu * x;
         mx = fmaxf(mx, isfinite(mux) ? fabsf(mux) : INFINITY);
         w += static_cast<double>(mu) * static_cast<double>(x);
@@ -127,7 +130,7 @@ __global__ __launch_bounds__(256) void k_colmax_part(int64_t rows, int64_t k,
       ws += shw[rr * kk + t];
     }
     part[(int64_t)blockIdx.x * k + c0 + t] = m;
-    if (mean) wpart[(int64_t)blockIdx.x * k + c0 + t] = ws;
+    if (with_mean) wpart[(int64_t)blockIdx.x * k + c0 + t] = ws;
   }
 }
 
@@ -213,11 +216,12 @@ __global__ __launch_bounds__(256) void k_quant_p1(const float* __restrict__ V, i
 }
 
 // Phase-2 B operand: Bu[l][nb][kb][2][lane][16]: half 0 = q(u), half 1 =
-// -q(mean*u); byte 4s+b holds slot kb*64 + 16g + 4b + s (slot-local rows).
+// q(mean*u) with the column's statistic set; byte 4s+b holds slot
+// kb*64 + 16g + 4b + s (slot-local rows).
 template <int L>
 __global__ __launch_bounds__(256) void k_quant_p2(const float* __restrict__ U, int64_t ldu,
                                                   int64_t nslots, int64_t k, int64_t KB,
-                                                  int64_t NBt, const float* __restrict__ mean,
+                                                  int64_t NBt, StatSets ss,
                                                   int64_t slot0, ChunkTable tab,
                                                   const double* __restrict__ inv_scale,
                                                   double qmax, uint8_t* __restrict__ Bu) {
@@ -235,6 +239,7 @@ __global__ __launch_bounds__(256) void k_quant_p2(const float* __restrict__ U, i
     int comp = 0;
     while (comp + 1 < tab.ncomp && sb_slot >= tab.end[comp]) ++comp;
     const double is = (c < k) ? inv_scale[(int64_t)comp * k + c] : 0.0;
+    const float* mean = ss.mean + (c < k ? ss.set_of(c) : 0) * ss.stride + slot0;
 #pragma unroll
     for (int s = 0; s < 4; ++s)
 #pragma unroll
@@ -242,7 +247,7 @@ __global__ __launch_bounds__(256) void k_quant_p2(const float* __restrict__ U, i
         const int64_t j = (kb / 4) * 256 + 64 * g + 16 * (kb % 4) + 4 * b + s;
         const bool ok = (c < k) && (j < nslots);
         const float u = ok ? U[j * ldu + c] : 0.0f;
-        const float mu = ok ? mean[slot0 + j] : 0.0f;
+        const float mu = ok ? mean[j] : 0.0f;
         int lu[3], lw[3];
         limbs<L>(quant(u, is, qmax), lu);
         limbs<L>(quant(mu * u, is, qmax), lw);
@@ -309,8 +314,7 @@ struct P1Args {
   int64_t k;
   const uint8_t* Bq;
   const double* scale; // per column
-  const float* mean;
-  const float* inv;
+  StatSets ss;
   CompScale cs;
   int inv_pow;
   float* U;            // (nslots, k) when nsplit == 1
@@ -459,9 +463,11 @@ __global__ __launch_bounds__(256) void k_p1_mfma(P1Args a) {
             A2 += ldexp(static_cast<double>(acco[l][mb][nb][v]), 8 * l);
           }
           const int64_t slot = a.slot0 + jl;
-          const double iv = a.inv[slot];
-          const double f = (a.inv_pow == 2 ? iv * iv : iv) * a.cs.scale_of(slot);
-          const double tv = a.scale[col] * (A1 - static_cast<double>(a.mean[slot]) * A2);
+          const int set = a.ss.set_of(col);
+          const int64_t so = set * a.ss.stride + slot;
+          const double iv = a.ss.inv[so];
+          const double f = (a.inv_pow == 2 ? iv * iv : iv) * a.ss.factor(set, a.cs, slot);
+          const double tv = a.scale[col] * (A1 - static_cast<double>(a.ss.mean[so]) * A2);
           a.U[jl * a.k + col] = static_cast<float>(f * tv);
         } else {
           int32_t* dst = a.part + ((wg.by * a.nslots + jl) * a.k + col) * (2 * L);
@@ -498,10 +504,12 @@ __global__ void k_p1_reduce(P1Args a) {
       A2 += ldexp(static_cast<double>(s2[l]), 8 * l);
     }
     const int64_t slot = a.slot0 + jl;
-    const double iv = a.inv[slot];
-    const double f = (a.inv_pow == 2 ? iv * iv : iv) * a.cs.scale_of(slot);
+    const int set = a.ss.set_of(col);
+    const int64_t so = set * a.ss.stride + slot;
+    const double iv = a.ss.inv[so];
+    const double f = (a.inv_pow == 2 ? iv * iv : iv) * a.ss.factor(set, a.cs, slot);
     a.U[jl * a.k + col] =
-        static_cast<float>(f * a.scale[col] * (A1 - static_cast<double>(a.mean[slot]) * A2));
+        static_cast<float>(f * a.scale[col] * (A1 - static_cast<double>(a.ss.mean[so]) * A2));
   }
 }
 
@@ -814,25 +822,27 @@ static ScaleView scale_view(const KvPlan& pl, int64_t k, uint8_t* ws) {
   return v;
 }
 
-// Column max |x| (and |mean * x| when mean is given) of rows [0, rows) of X
-// -> scale[c] = max / qmax, inv_scale[c] = 1 / scale[c]; with mean, also
+// Column max |x| (and |mean * x| when with_mean) of rows [0, rows) of X
+// -> scale[c] = max / qmax, inv_scale[c] = 1 / scale[c]; with the mean
+// (column c's statistic set, rows mean_off + i), also
 // wsum[c] = sum_i mean_i x_ic in FP64.
-static int colscale(const float* X, int64_t rows, int64_t k, int64_t ldx, const float* mean,
-                    int64_t mean_off, double qmax, double* scale, double* inv_scale, double* wsum,
-                    float* part, double* wpart, hipStream_t st) {
+static int colscale(const float* X, int64_t rows, int64_t k, int64_t ldx, bool with_mean,
+                    const StatSets& ss, int64_t mean_off, double qmax, double* scale,
+                    double* inv_scale, double* wsum, float* part, double* wpart, hipStream_t st) {
   const int nblk = (int)std::max<int64_t>(1, std::min<int64_t>(kColmaxBlocks, ceil_div(rows * std::min<int64_t>(k, 256), 4096)));
   hipLaunchKernelGGL(k_colmax_part, dim3(nblk, (unsigned)ceil_div(k, 256)), dim3(256), 0, st, rows,
-                     k, X, ldx, mean, mean_off, part, wpart);
+                     k, X, ldx, with_mean ? 1 : 0, ss, mean_off, part, wpart);
   COH_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_colmax_final, dim3((unsigned)std::min<int64_t>(k, 1024)), dim3(256), 0, st,
-                     part, wpart, nblk, k, qmax, scale, inv_scale, mean ? wsum : nullptr);
+                     part, wpart, nblk, k, qmax, scale, inv_scale, with_mean ? wsum : nullptr);
   COH_CHECK_LAUNCH();
   return COH_OK;
 }
 
 template <int L>
 static int phase1(const coh_panel* p, const KvPlan& pl, const float* V, int64_t k, int64_t ldv,
-                  const CompScale& cs, int inv_pow, uint8_t* ws, hipStream_t st) {
+                  const CompScale& cs, const StatSets& ss, int inv_pow, uint8_t* ws,
+                  hipStream_t st) {
   const double qmax = (L == 3) ? kQ3 : kQ2;
   const int64_t nslots = pl.slot1 - pl.slot0;
   const int NB = pl.shape[0], MB = pl.shape[1], T = pl.shape[2], PD = pl.shape[3];
@@ -840,7 +850,7 @@ static int phase1(const coh_panel* p, const KvPlan& pl, const float* V, int64_t 
   const int64_t groups = NBt / NB;
   const int64_t KB = pl.n_pad / 64;
   const ScaleView sv = scale_view(pl, k, ws);
-  int s = colscale(V, p->n, k, ldv, nullptr, 0, qmax, sv.scale, sv.inv_scale, nullptr, sv.part,
+  int s = colscale(V, p->n, k, ldv, false, ss, 0, qmax, sv.scale, sv.inv_scale, nullptr, sv.part,
                    nullptr, st);
   if (s != COH_OK) return s;
   const double* scale = sv.scale;
@@ -860,8 +870,7 @@ static int phase1(const coh_panel* p, const KvPlan& pl, const float* V, int64_t 
   a.k = k;
   a.Bq = Bq;
   a.scale = scale;
-  a.mean = p->mean;
-  a.inv = p->inv;
+  a.ss = ss;
   a.cs = cs;
   a.inv_pow = inv_pow;
   a.U = reinterpret_cast<float*>(ws + pl.off_U);
@@ -901,8 +910,9 @@ static int phase1(const coh_panel* p, const KvPlan& pl, const float* V, int64_t 
 }
 
 template <int L>
-static int phase2(const coh_panel* p, const KvPlan& pl, int64_t k, int stacked, float* out,
-                  int64_t ldo, int64_t out_stride, float beta, uint8_t* ws, hipStream_t st) {
+static int phase2(const coh_panel* p, const KvPlan& pl, int64_t k, const StatSets& ss, int stacked,
+                  float* out, int64_t ldo, int64_t out_stride, float beta, uint8_t* ws,
+                  hipStream_t st) {
   const double qmax = (L == 3) ? kQ3 : kQ2;
   const int64_t nslots = pl.slot1 - pl.slot0;
   const int NB = pl.shape[4], MB = pl.shape[5], T = pl.shape[6], PD = pl.shape[7];
@@ -917,7 +927,7 @@ static int phase2(const coh_panel* p, const KvPlan& pl, int64_t k, int stacked, 
   for (int g = 0; g < pl.tab.ncomp; ++g) {
     const int64_t r0 = pl.tab.begin[g] - pl.slot0;
     const int64_t rows = pl.tab.end[g] - pl.tab.begin[g];
-    int s = colscale(U + r0 * pl.ldu, rows, k, pl.ldu, p->mean, pl.tab.begin[g], qmax,
+    int s = colscale(U + r0 * pl.ldu, rows, k, pl.ldu, true, ss, pl.tab.begin[g], qmax,
                      sv.scale + g * k, sv.inv_scale + g * k, sv.wsum + g * k, sv.part, sv.wpart,
                      st);
     if (s != COH_OK) return s;
@@ -926,7 +936,7 @@ static int phase2(const coh_panel* p, const KvPlan& pl, int64_t k, int stacked, 
   const double* inv_scale = sv.inv_scale;
   uint8_t* Bu = ws + pl.off_p2;
   hipLaunchKernelGGL(k_quant_p2<L>, dim3(grid1d(NBt * KB * 64)), dim3(256), 0, st, U, pl.ldu,
-                     nslots, k, KB, NBt, p->mean, pl.slot0, pl.tab, inv_scale, qmax, Bu);
+                     nslots, k, KB, NBt, ss, pl.slot0, pl.tab, inv_scale, qmax, Bu);
   COH_CHECK_LAUNCH();
   P2Args a;
   a.smaj = p->smaj;
@@ -975,15 +985,17 @@ static int phase2(const coh_panel* p, const KvPlan& pl, int64_t k, int stacked, 
 }
 
 int run_phase1_mfma(const coh_panel* p, const KvPlan& pl, const float* V, int64_t k, int64_t ldv,
-                    const CompScale& cs, int inv_pow, uint8_t* ws, hipStream_t st) {
-  return pl.limbs == 3 ? phase1<3>(p, pl, V, k, ldv, cs, inv_pow, ws, st)
-                       : phase1<2>(p, pl, V, k, ldv, cs, inv_pow, ws, st);
+                    const CompScale& cs, const StatSets& ss, int inv_pow, uint8_t* ws,
+                    hipStream_t st) {
+  return pl.limbs == 3 ? phase1<3>(p, pl, V, k, ldv, cs, ss, inv_pow, ws, st)
+                       : phase1<2>(p, pl, V, k, ldv, cs, ss, inv_pow, ws, st);
 }
 
-int run_phase2_mfma(const coh_panel* p, const KvPlan& pl, int64_t k, int stacked, float* out,
-                    int64_t ldo, int64_t out_stride, float beta, uint8_t* ws, hipStream_t st) {
-  return pl.limbs == 3 ? phase2<3>(p, pl, k, stacked, out, ldo, out_stride, beta, ws, st)
-                       : phase2<2>(p, pl, k, stacked, out, ldo, out_stride, beta, ws, st);
+int run_phase2_mfma(const coh_panel* p, const KvPlan& pl, int64_t k, const StatSets& ss,
+                    int stacked, float* out, int64_t ldo, int64_t out_stride, float beta,
+                    uint8_t* ws, hipStream_t st) {
+  return pl.limbs == 3 ? phase2<3>(p, pl, k, ss, stacked, out, ldo, out_stride, beta, ws, st)
+                       : phase2<2>(p, pl, k, ss, stacked, out, ldo, out_stride, beta, ws, st);
 }
 
 }  // namespace coh

@@ -18,7 +18,39 @@ struct CompScale {
       if (slot < end[g]) return scale[g];
     return 0.0f;
   }
+  __device__ int comp_of(int64_t slot) const {
+    int g = 0;
+    while (g + 1 < ncomp && slot >= end[g]) ++g;
+    return g;
+  }
 };
+
+// Standardization statistics of an operator call.  Either the panel's own
+// (one set, col_set == null, comp_scale == null: CompScale applies), or one
+// set per sample subset with a set index per right-hand column: column c
+// uses mean/inv[set * stride + slot] and component factor
+// comp_scale[set * ncomp + g] (device arrays).  Set indices are clamped to
+// [0, nsets).
+struct StatSets {
+  const float* mean;
+  const float* inv;
+  int64_t stride;
+  int nsets;
+  const int32_t* col_set;
+  const float* comp_scale;
+  __device__ int set_of(int64_t col) const {
+    if (col_set == nullptr) return 0;
+    const int t = col_set[col];
+    return t < 0 ? 0 : (t >= nsets ? nsets - 1 : t);
+  }
+  __device__ float factor(int set, const CompScale& cs, int64_t slot) const {
+    return comp_scale ? comp_scale[set * cs.ncomp + cs.comp_of(slot)] : cs.scale_of(slot);
+  }
+};
+
+inline StatSets panel_stats(const coh_panel* p) {
+  return StatSets{p->mean, p->inv, 0, 1, nullptr, nullptr};
+}
 
 // Slot chunks of the phase-2 contraction; a chunk never straddles components.
 struct ChunkTable {
@@ -73,6 +105,7 @@ int reduce_p2_partials(const KvPlan& pl, const double* part, int64_t n, int64_t 
 
 // plain FP32 on the matrix cores (kv_f32.hip)
 int plan_f32(const coh_panel* p, int64_t k, int32_t cb, int32_t ce, KvPlan* pl);
+// (panel statistics only)
 int run_phase1_f32(const coh_panel* p, const KvPlan& pl, const float* V, int64_t k, int64_t ldv,
                    const CompScale& cs, int inv_pow, uint8_t* ws, hipStream_t st);
 int run_phase2_f32(const coh_panel* p, const KvPlan& pl, int64_t k, int stacked, float* out,
@@ -81,9 +114,11 @@ int run_phase2_f32(const coh_panel* p, const KvPlan& pl, int64_t k, int stacked,
 // MFMA int8-limb path (kv_mfma.hip)
 int plan_mfma(const coh_panel* p, int precision, int64_t k, int32_t cb, int32_t ce, KvPlan* pl);
 int run_phase1_mfma(const coh_panel* p, const KvPlan& pl, const float* V, int64_t k, int64_t ldv,
-                    const CompScale& cs, int inv_pow, uint8_t* ws, hipStream_t st);
-int run_phase2_mfma(const coh_panel* p, const KvPlan& pl, int64_t k, int stacked, float* out,
-                    int64_t ldo, int64_t out_stride, float beta, uint8_t* ws, hipStream_t st);
+                    const CompScale& cs, const StatSets& ss, int inv_pow, uint8_t* ws,
+                    hipStream_t st);
+int run_phase2_mfma(const coh_panel* p, const KvPlan& pl, int64_t k, const StatSets& ss,
+                    int stacked, float* out, int64_t ldo, int64_t out_stride, float beta,
+                    uint8_t* ws, hipStream_t st);
 
 inline int make_plan(const coh_panel* p, int precision, int64_t k, int32_t cb, int32_t ce,
                      KvPlan* pl) {
@@ -98,21 +133,23 @@ inline int make_plan(const coh_panel* p, int precision, int64_t k, int32_t cb, i
 inline int run_xtv(const coh_panel* p, const KvPlan& pl, int precision, const float* V,
                    int64_t k, int64_t ldv, const CompScale& cs, uint8_t* ws, hipStream_t st) {
   if (precision == COH_PREC_FP32) return run_phase1_f32(p, pl, V, k, ldv, cs, 1, ws, st);
-  return run_phase1_mfma(p, pl, V, k, ldv, cs, 1, ws, st);
+  return run_phase1_mfma(p, pl, V, k, ldv, cs, panel_stats(p), 1, ws, st);
 }
 
+// K.V with the panel's statistics (any precision) or with statistic sets
+// (int8-limb precisions only; the caller checks).
 inline int run_kv(const coh_panel* p, const KvPlan& pl, int precision, const float* V, int64_t k,
-                  int64_t ldv, const CompScale& cs, int stacked, float* out, int64_t ldo,
-                  int64_t out_stride, float beta, uint8_t* ws, hipStream_t st) {
+                  int64_t ldv, const CompScale& cs, const StatSets& ss, int stacked, float* out,
+                  int64_t ldo, int64_t out_stride, float beta, uint8_t* ws, hipStream_t st) {
   int s;
   if (precision == COH_PREC_FP32) {
     s = run_phase1_f32(p, pl, V, k, ldv, cs, 2, ws, st);
     if (s != COH_OK) return s;
     return run_phase2_f32(p, pl, k, stacked, out, ldo, out_stride, beta, ws, st);
   }
-  s = run_phase1_mfma(p, pl, V, k, ldv, cs, 2, ws, st);
+  s = run_phase1_mfma(p, pl, V, k, ldv, cs, ss, 2, ws, st);
   if (s != COH_OK) return s;
-  return run_phase2_mfma(p, pl, k, stacked, out, ldo, out_stride, beta, ws, st);
+  return run_phase2_mfma(p, pl, k, ss, stacked, out, ldo, out_stride, beta, ws, st);
 }
 
 }  // namespace coh

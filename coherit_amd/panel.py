@@ -59,6 +59,7 @@ class ResidentPanel:
         cs = np.zeros(self.n_calls, np.int64)
         N.call("coh_panel_call_slots", self._h, N.ptr(cs))
         self.call_slot = cs
+        self.call_starts = starts
         self.call_widths = widths
         self.call_components = comps
         self.finalized = False
@@ -134,10 +135,24 @@ class ResidentPanel:
                N.ptr(s2), N.stream_ptr())
         return tuple(t.cpu().numpy() for t in (mean, inv, cnt, s1, s2))
 
+    @property
+    def cache_to_slot_host(self) -> np.ndarray:
+        """(m,) int64 slot of every cache-order SNP."""
+        if getattr(self, "_c2s", None) is None:
+            c2s = np.empty(self.m, np.int64)
+            for c in range(self.n_calls):
+                a, w = int(self.call_starts[c]), int(self.call_widths[c])
+                c2s[a:a + w] = int(self.call_slot[c]) + np.arange(w)
+            self._c2s = c2s
+        return self._c2s
+
     def set_standardization(self, mean, inv_sd) -> None:
         """Replace mean / inv_sd (cache order, m entries) of the finalized panel."""
-        mo = torch.as_tensor(np.asarray(mean, np.float32).reshape(-1), device=self.device)
-        io = torch.as_tensor(np.asarray(inv_sd, np.float32).reshape(-1), device=self.device)
+        def dev(x):
+            if isinstance(x, torch.Tensor):
+                return x.to(device=self.device, dtype=torch.float32).reshape(-1).contiguous()
+            return torch.as_tensor(np.asarray(x, np.float32).reshape(-1), device=self.device)
+        mo, io = dev(mean), dev(inv_sd)
         if mo.numel() != self.m or io.numel() != self.m:
             raise ValueError(f"standardization needs {self.m} entries per array")
         N.call("coh_panel_set_standardization", self.handle, N.ptr(mo), N.ptr(io), N.stream_ptr())

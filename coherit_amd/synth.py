@@ -125,6 +125,41 @@ def phenotype(streamer, wl: Workload, *, seed: int | None = None):
     return y.astype(np.float32), X.astype(np.float32)
 
 
+def trait_phenotypes(streamer, wl: Workload, n_traits: int, *, seed: int | None = None,
+                     max_mask: float = 0.3, h2_range=(0.1, 0.7)):
+    """Multi-trait phenotypes on one panel: (Y (n, T), X (n, p+1), keep (T, n), h2 (T,)).
+
+    Trait t has total h2_t ~ U(h2_range), split over the components in the
+    proportions of ``wl.h2``; its genetic term per component is Z_g b_g with
+    1% causal SNPs (one Z.B pass per component for all traits).  Each trait
+    keeps samples outside a missing-phenotype mask covering a U(0, max_mask)
+    fraction; covariates (intercept + ``wl.n_covar`` N(0,1)) are shared.
+    """
+    rng = np.random.default_rng(wl.seed + 11 if seed is None else seed)
+    n, T = streamer.n, int(n_traits)
+    X = np.concatenate([np.ones((n, 1)), rng.standard_normal((n, wl.n_covar))], axis=1)
+    h2 = rng.uniform(h2_range[0], h2_range[1], size=T)
+    share = np.asarray(wl.h2, np.float64) / float(np.sum(wl.h2))
+    off = streamer._component_snp_offsets
+    Y = X @ rng.standard_normal((wl.n_covar + 1, T))
+    for g in range(streamer.n_components):
+        a, b = int(off[g]), int(off[g + 1])
+        size = b - a
+        k = max(1, int(round(0.01 * size)))
+        B = np.zeros((streamer.m, T), np.float32)
+        for t in range(T):
+            causal = rng.choice(size, size=k, replace=False) + a
+            B[causal, t] = rng.standard_normal(k)
+        out = streamer.panel.zb(torch.as_tensor(B), 0, streamer.n_components, precision=0)
+        ZB = out.double().cpu().numpy()
+        ZB = (ZB - ZB.mean(0)) / np.maximum(ZB.std(0), 1e-12)
+        Y += np.sqrt(h2 * share[g])[None, :] * ZB
+    Y += rng.standard_normal((n, T)) * np.sqrt(np.maximum(1e-6, 1.0 - h2))[None, :]
+    frac = rng.uniform(0.0, max_mask, size=T)
+    keep = rng.random((T, n)) >= frac[:, None]
+    return Y.astype(np.float32), X.astype(np.float32), keep, h2
+
+
 def initial_components(y, X, atoms, h2_init: float = 0.5) -> np.ndarray:
     """Start point (theta_g..., theta_e) scaled to the OLS residual variance.
 

@@ -116,6 +116,18 @@ int coh_panel_copy_stats(const coh_panel* panel, float* dev_mean, float* dev_inv
  * `_copy_training_standardization_to_test_streamer`). */
 int coh_panel_set_standardization(coh_panel* panel, const float* dev_mean, const float* dev_inv,
                                   void* stream);
+/* Statistics of sample subsets of a finalized panel, as a streamer built
+ * with that sample_mask computes them (geno_stream.py:1161-1247, 2597-2598;
+ * float32 stats geno_stream.py:1780-1792).  dev_keep[s * n + i] != 0 keeps
+ * sample i in subset s.  Outputs are slot-indexed device arrays at
+ * [s * stride + slot] (stride >= m_slot): float32 mean and inv_sd, and the
+ * kept observed-call counts in dev_cnt unless it is NULL; padding slots get
+ * zeros.  Computed from the packed codes, which equals the reference when the
+ * raw values are 0/1/2 or missing (always for BED and packed input). */
+int coh_subset_stats_workspace_bytes(const coh_panel* panel, int32_t nsets, uint64_t* bytes);
+int coh_panel_subset_stats(const coh_panel* panel, int32_t nsets, const uint8_t* dev_keep,
+                           float* dev_mean, float* dev_inv, int64_t* dev_cnt, int64_t stride,
+                           void* dev_workspace, uint64_t workspace_bytes, void* stream);
 /* Read one call back in the reference cache layout (n, ceil(w/4)). */
 int coh_panel_read_packed_call(const coh_panel* panel, int32_t call_idx, uint8_t* dev_out,
                                int64_t ld, void* stream);
@@ -142,6 +154,26 @@ int coh_kv(const coh_panel* panel, int32_t precision, const float* dev_V, int64_
            int32_t comp_begin, int32_t comp_end, const float* host_comp_scale, int32_t stacked,
            float* dev_out, int64_t ldo, int64_t out_stride, float beta, void* dev_workspace,
            uint64_t workspace_bytes, void* stream);
+
+/* K.V with one statistic set per right-hand column: the batched multi-trait
+ * form, in which traits with their own missing-phenotype masks share every
+ * pass over the panel.  Column c is standardized with set s = dev_col_set[c]
+ * (mean/inv_sd at [s * set_stride + slot], slot-indexed, as written by
+ * coh_panel_subset_stats; indices are clamped to [0, nsets)) and scaled per
+ * component by dev_set_comp_scale[s * (comp_end - comp_begin) + g] (e.g.
+ * 1/eff of component g under subset s).  With V's rows outside subset s
+ * zero, column c restricted to the subset's rows is what a streamer built
+ * with that sample_mask (geno_stream.py:1161-1247) returns from the same
+ * operator call (kv_impl.py:1042-1187).  No counterpart in the reference,
+ * which runs one fit per mask; int8-limb precisions only (the plain FP32
+ * path takes one set through coh_panel_set_standardization).  Workspace:
+ * coh_workspace_bytes(COH_OP_KV). */
+int coh_kv_sets(const coh_panel* panel, int32_t precision, const float* dev_V, int64_t k,
+                int64_t ldv, int32_t comp_begin, int32_t comp_end, int32_t nsets,
+                const float* dev_set_mean, const float* dev_set_inv, int64_t set_stride,
+                const int32_t* dev_col_set, const float* dev_set_comp_scale, int32_t stacked,
+                float* dev_out, int64_t ldo, int64_t out_stride, float beta,
+                void* dev_workspace, uint64_t workspace_bytes, void* stream);
 
 /* out = scale * Z^T V for components [comp_begin, comp_end)
  * (kv_impl.py:163-185, 382-439).  slot_space == 0: rows in cache order

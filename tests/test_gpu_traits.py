@@ -1,0 +1,218 @@
+"""GPU: the batched multi-trait form.
+
+Each trait is a fit on its own sample subset (the reference's
+``sample_mask`` streamer, geno_stream.py:1161-1247).  Checked here:
+subset statistics bit-exact against the oracle panel of the subset's
+samples; the statistic-set operator against each subset's dense FP64
+operator (the form's tolerance) and against a streamer built on the subset
+alone (agreement to FP32 rounding); batched fits against the same traits
+fitted alone (|dh2| <= 1e-5) and against the FP64 oracle fit on each subset
+(|dh2| <= 1e-3 for the split form, 1e-5 for the plain trait-by-trait path).
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import geno
+from oracle import reml as oreml
+from tests.helpers import int8_block, random_codes
+
+pytestmark = pytest.mark.gpu
+
+
+def _streamer(codes, sizes, precision="i8x2", call_width=400):
+    from coherit_amd.geno_stream import GenoBlockStreamer
+    from coherit_amd.sources import ArraySource
+
+    return GenoBlockStreamer(ArraySource(int8_block(codes), missing_val=-9), call_width=call_width,
+                             component_block_sizes=sizes, precision=precision)
+
+
+def _keep(n, fracs, seed):
+    rng = np.random.default_rng(seed)
+    return np.stack([rng.random(n) >= f for f in fracs])
+
+
+def _dense_stack(o):
+    Z = o.standardized()
+    off = o.component_offsets
+    return [Z[:, off[g]:off[g + 1]] @ Z[:, off[g]:off[g + 1]].T / float(o.atoms.component_eff[g])
+            for g in range(len(o.component_sizes))]
+
+
+def _bits(x):
+    return np.asarray(x, np.float32).view(np.int32)
+
+
+def test_subset_statistics_are_bit_exact_against_oracle_subsets():
+    from coherit_amd.traits import SubsetStats
+
+    n, m = 777, 1100
+    sizes = [300, 800]
+    codes = random_codes(n, m, seed=41, miss=0.05)
+    rng = np.random.default_rng(4)
+    keep = [np.ones(n, bool), rng.random(n) >= 0.3, np.arange(n) % 2 == 0,
+            np.arange(n) < 17, codes[:, 0] == 3]
+    keep = np.stack(keep)
+    st = _streamer(codes, sizes, "fp32")
+    stats = SubsetStats(st.panel, keep)
+    c2s = st.panel.cache_to_slot_host
+    mean = stats.mean.cpu().numpy()[:, c2s]
+    inv = stats.inv.cpu().numpy()[:, c2s]
+    for t in range(keep.shape[0]):
+        o = geno.OraclePanel.from_codes(codes[keep[t]], sizes)
+        print("subset", t, "n_t", int(keep[t].sum()), flush=True)
+        np.testing.assert_array_equal(_bits(mean[t]), _bits(o.mean))
+        np.testing.assert_array_equal(_bits(inv[t]), _bits(o.inv_sd))
+        np.testing.assert_array_equal(stats.eff[t], np.asarray(o.atoms.component_eff, np.float32))
+        np.testing.assert_allclose(stats.atoms[t], o.atoms.component_mean_diag, rtol=1e-6)
+    # the all-missing SNP of the last subset is invalid there
+    assert inv[4, 0] == 0.0 and mean[4, 0] == 0.0
+    # the full-sample subset equals the panel's own statistics
+    own = st.panel.stats()
+    np.testing.assert_array_equal(_bits(mean[0]), _bits(own[0]))
+    np.testing.assert_array_equal(_bits(inv[0]), _bits(own[1]))
+    st.close()
+
+
+@pytest.mark.parametrize("precision,tol", [("i8x3", 1e-5), ("i8x2", 2e-4)])
+def test_statistic_set_operator_matches_each_subset(precision, tol):
+    from coherit_amd.traits import PassBatcher, SubsetStats, _Request
+
+    n, m = 400, 1300
+    sizes = [500, 800]
+    codes = random_codes(n, m, seed=42)
+    keep = _keep(n, (0.0, 0.25, 0.4), seed=5)
+    st = _streamer(codes, sizes, precision)
+    stats = SubsetStats(st.panel, keep)
+    batcher = PassBatcher(st, stats, 1)
+    rng = np.random.default_rng(6)
+    ks = (5, 12, 20)
+    reqs = []
+    for t, k in enumerate(ks):
+        V = torch.as_tensor(rng.standard_normal((int(keep[t].sum()), k)).astype(np.float32),
+                            device="cuda")
+        reqs.append(_Request(t, V, precision))
+    batcher._contract(reqs, precision)
+    torch.cuda.synchronize()
+    for t, r in enumerate(reqs):
+        o = geno.OraclePanel.from_codes(codes[keep[t]], sizes)
+        Ks = _dense_stack(o)
+        V = r.V.double().cpu().numpy()
+        got = r.result.double().cpu().numpy()
+        for g in range(len(sizes)):
+            want = Ks[g] @ V
+            err = np.linalg.norm(got[g] - want, axis=0) / np.linalg.norm(want, axis=0)
+            print(precision, "trait", t, "comp", g, "max col rel err", float(err.max()), flush=True)
+            assert float(err.max()) <= tol
+        # the same product from a streamer built on the subset alone
+        solo = _streamer(codes[keep[t]], sizes, precision)
+        ref = solo.stacked_component_kv(r.V).double().cpu().numpy()
+        d = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+        print(precision, "trait", t, "batched vs solo", d, flush=True)
+        assert d <= 1e-6
+        solo.close()
+    st.close()
+
+
+def _pheno(codes, T, seed):
+    rng = np.random.default_rng(seed)
+    n = codes.shape[0]
+    o = geno.OraclePanel.from_codes(codes, None)
+    Z = o.standardized()
+    X = np.concatenate([np.ones((n, 1)), rng.standard_normal((n, 2))], axis=1)
+    Y = np.empty((n, T))
+    for t in range(T):
+        h2 = 0.2 + 0.15 * t
+        zb = Z @ rng.standard_normal(Z.shape[1])
+        Y[:, t] = (X @ rng.standard_normal(3) + np.sqrt(h2) * zb / zb.std()
+                   + np.sqrt(1 - h2) * rng.standard_normal(n))
+    return Y.astype(np.float32), X.astype(np.float32)
+
+
+def _h2(theta, atoms):
+    G = len(atoms)
+    gen = float(np.dot(theta[:G], atoms))
+    return gen / (gen + float(theta[G]))
+
+
+def _oracle_fit(codes, keep_t, sizes, y, X, U, kw, param_init=None):
+    o = geno.OraclePanel.from_codes(codes[keep_t], sizes)
+    pb = oreml.Problem(_dense_stack(o), o.atoms.component_mean_diag.astype(np.float64))
+    th = oreml.fit_reml(pb, y, X, seed=kw["seed"], slq_samples=kw["slq_samples"], slq_m=kw["slq_m"],
+                        minq_iter=kw["minq_iter"], pcg_tol=kw["reml_pcg_tol"],
+                        slq_mode="projected_core_residual", precond_U=U, param_init=param_init)
+    return th, o.atoms.component_mean_diag.astype(np.float64)
+
+
+def test_batched_fits_match_solo_fits_and_oracle():
+    from coherit_amd.reml_model import FitConfig, InfinitesimalREMLFitter
+    from coherit_amd.traits import fit_traits
+
+    n, m = 360, 1400
+    sizes = [560, 840]
+    codes = random_codes(n, m, seed=43)
+    T = 3
+    keep = _keep(n, (0.0, 0.15, 0.3), seed=8)
+    Y, X = _pheno(codes, T, seed=9)
+    kw = dict(precond_rank=40, slq_samples=8, slq_m=12, minq_iter=12, reml_pcg_tol=1e-5, seed=7)
+    cfg = FitConfig(call_width=400, verbose=False, precision="i8x2", **kw)
+    st = _streamer(codes, sizes, "i8x2")
+    res, info = fit_traits(st, Y, keep, X, cfg=cfg, keep_precond=True)
+    print("batched info", {k: v for k, v in info.items() if k != "precond_U"}, flush=True)
+    # the traits' operator calls shared passes: several requests per batch
+    assert info["requests"] >= 2 * info["batches"]
+    for t in range(T):
+        idx = np.flatnonzero(keep[t])
+        y, Xt = Y[idx, t], X[idx]
+        solo_st = _streamer(codes[keep[t]], sizes, "i8x2")
+        fitter = InfinitesimalREMLFitter(cfg, streamers=[solo_st])
+        solo = fitter.fit_infinitesimal(y, Xt)
+        th_o, atoms = _oracle_fit(codes, keep[t], sizes, y, Xt,
+                                  info["precond_U"][t].double().cpu().numpy(), kw)
+        print("trait", t, "batched", res[t].var_components, res[t].heritability, "solo",
+              solo.var_components, solo.heritability, "oracle", th_o, _h2(th_o, atoms), flush=True)
+        assert abs(res[t].heritability - solo.heritability) <= 1e-5
+        np.testing.assert_allclose(res[t].var_components, solo.var_components, rtol=1e-4,
+                                   atol=1e-6)
+        assert abs(res[t].heritability - _h2(th_o, atoms)) <= 1e-3
+        fitter.close()
+    st.close()
+
+
+def test_plain_trait_by_trait_fits_match_oracle():
+    from coherit_amd import synth
+    from coherit_amd.reml_model import FitConfig
+    from coherit_amd.traits import SubsetStats, fit_traits_sequential
+
+    n, m = 320, 1200
+    sizes = [500, 700]
+    codes = random_codes(n, m, seed=44)
+    keep = _keep(n, (0.1, 0.25), seed=10)
+    Y, X = _pheno(codes, 2, seed=11)
+    kw = dict(precond_rank=40, slq_samples=8, slq_m=12, minq_iter=12, reml_pcg_tol=1e-5, seed=7)
+    cfg = FitConfig(call_width=400, verbose=False, precision="fp32", **kw)
+    st = _streamer(codes, sizes, "fp32")
+    own = st.panel.stats()
+    stats = SubsetStats(st.panel, keep)
+    # start at the OLS-residual scale so that the fits take accepted steps
+    init = np.stack([synth.initial_components(Y[keep[t], t], X[keep[t]], stats.atoms[t])
+                     for t in range(2)])
+    res, info = fit_traits_sequential(st, Y, keep, X, cfg=cfg, var_components_init=init,
+                                      stats=stats, keep_precond=True)
+    for t in range(2):
+        idx = np.flatnonzero(keep[t])
+        th_o, atoms = _oracle_fit(codes, keep[t], sizes, Y[idx, t], X[idx],
+                                  info["precond_U"][t].double().cpu().numpy(), kw,
+                                  param_init=init[t])
+        print("trait", t, res[t].var_components, "oracle", th_o, "init", init[t],
+              [h["status"] for h in res[t].history], flush=True)
+        assert any(h["status"] == "accept" for h in res[t].history)
+        assert not np.allclose(res[t].var_components, init[t], rtol=1e-3)
+        assert abs(res[t].heritability - _h2(th_o, atoms)) <= 1e-5
+        np.testing.assert_allclose(res[t].var_components, th_o, rtol=1e-4, atol=1e-7)
+    # the panel's own statistics are restored
+    after = st.panel.stats()
+    np.testing.assert_array_equal(_bits(after[0]), _bits(own[0]))
+    np.testing.assert_array_equal(_bits(after[1]), _bits(own[1]))
+    st.close()
