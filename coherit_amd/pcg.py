@@ -9,6 +9,7 @@ FP64, axpys in FP32): ``coh_pcg_state_update`` / ``coh_pcg_direction_update``.
 from __future__ import annotations
 
 import ctypes
+import threading
 import math
 from typing import Callable, Optional
 
@@ -16,17 +17,23 @@ import torch
 
 from . import _native as N
 
-_WS_CACHE: dict = {}
+# Reduction workspaces, one set per host thread: the partial sums of one
+# reduction and its final pass are separate launches, and solves running in
+# other threads (concurrent trait fits) enqueue theirs in between.
+_WS_LOCAL = threading.local()
 
 
 def _red_ws(k: int, device) -> tuple[torch.Tensor, int]:
     nbytes = ctypes.c_uint64()
     N.call("coh_reduce_workspace_bytes", 1, int(k), ctypes.byref(nbytes))
+    cache = getattr(_WS_LOCAL, "cache", None)
+    if cache is None:
+        cache = _WS_LOCAL.cache = {}
     key = (device, int(nbytes.value))
-    ws = _WS_CACHE.get(key)
+    ws = cache.get(key)
     if ws is None:
         ws = torch.empty(int(nbytes.value), dtype=torch.uint8, device=device)
-        _WS_CACHE[key] = ws
+        cache[key] = ws
     return ws, int(nbytes.value)
 
 

@@ -13,6 +13,7 @@ so the re-orthogonalisation is a batched GEMM over probes.
 from __future__ import annotations
 
 import ctypes
+import threading
 
 import numpy as np
 import torch
@@ -63,7 +64,9 @@ def _on_device(P, w) -> bool:
     return P.is_cuda and P.dtype == torch.float32 and w.dtype == torch.float32
 
 
-_DOTS_WS: dict = {}
+# Partial-sum workspace of coh_probe_dots, one per host thread (concurrent
+# trait fits enqueue their own dot products between the two launches).
+_DOTS_LOCAL = threading.local()
 
 
 def _probe_dots(P, x):
@@ -74,11 +77,10 @@ def _probe_dots(P, x):
     nbytes = ctypes.c_uint64()
     N.call("coh_probe_dots_workspace_bytes", S, r, n, ctypes.byref(nbytes))
     key = (P.device, int(nbytes.value))
-    ws = _DOTS_WS.get(key)
+    ws = getattr(_DOTS_LOCAL, "ws", {}).get(key)
     if ws is None:
-        _DOTS_WS.clear()
         ws = torch.empty(max(int(nbytes.value), 256), dtype=torch.uint8, device=P.device)
-        _DOTS_WS[key] = ws
+        _DOTS_LOCAL.ws = {key: ws}
     N.call("coh_probe_dots", S, r, n, N.ptr(P), P.stride(0), P.stride(1), N.ptr(x), x.stride(0),
            N.ptr(c), c.stride(0), N.ptr(ws), int(nbytes.value), N.stream_ptr())
     return c

@@ -392,9 +392,13 @@ def fit_traits(streamer, Y, keep, covar=None, *, cfg: FitConfig | None = None,
     for th in threads:
         th.join()
     wall = time.perf_counter() - t0
-    for e in errors:
+    for t, e in enumerate(errors):
         if e is not None:
-            raise e
+            try:
+                named = type(e)(f"trait {t}: {e}")
+            except Exception:
+                raise e
+            raise named from e
     info = dict(wall_s=wall, batches=batcher.n_batches, kv_sets_calls=batcher.n_calls,
                 columns=batcher.columns, requests=batcher.requests, contract_s=batcher.busy_s)
     if keep_precond:

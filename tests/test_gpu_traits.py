@@ -216,3 +216,47 @@ def test_plain_trait_by_trait_fits_match_oracle():
     np.testing.assert_array_equal(_bits(after[0]), _bits(own[0]))
     np.testing.assert_array_equal(_bits(after[1]), _bits(own[1]))
     st.close()
+
+
+def test_concurrent_solves_in_threads_match_serial_solves():
+    """Concurrent fits share one stream: every two-launch reduction must use
+    its own thread's workspace, or the threads' partial sums interleave."""
+    import threading
+
+    from coherit_amd.pcg import pcg_solve
+    from coherit_amd.slq import _probe_dots
+
+    rng = np.random.default_rng(12)
+    n, k, T = 3000, 6, 8
+    probs = []
+    for t in range(T):
+        A = rng.standard_normal((n, 40)).astype(np.float32)
+        d = rng.uniform(1.0, 2.0, n).astype(np.float32)
+        B = rng.standard_normal((n, k)).astype(np.float32)
+        P = rng.standard_normal((4, 30, n)).astype(np.float32)
+        probs.append(tuple(torch.as_tensor(x, device="cuda") for x in (A, d, B, P)))
+
+    def solve(t):
+        A, d, B, P = probs[t]
+        X, res, it = pcg_solve(lambda V: d[:, None] * V + A @ (A.T @ V), B, tol=1e-6, maxiter=500)
+        c = _probe_dots(P, B[:, :4].T.contiguous())
+        return X, it, c
+
+    serial = [solve(t) for t in range(T)]
+    torch.cuda.synchronize()
+    out = [None] * T
+
+    def run(t):
+        for _ in range(3):
+            out[t] = solve(t)
+
+    th = [threading.Thread(target=run, args=(t,)) for t in range(T)]
+    for x in th:
+        x.start()
+    for x in th:
+        x.join(timeout=120)
+    torch.cuda.synchronize()
+    for t in range(T):
+        assert out[t][1] == serial[t][1]
+        assert torch.equal(out[t][0], serial[t][0])
+        assert torch.equal(out[t][2], serial[t][2])
