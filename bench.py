@@ -54,6 +54,8 @@ def parse():
     ap.add_argument("--plain-traits", type=int, default=1,
                     help="batched: traits fitted one by one on the plain FP32 kernels (and on "
                          "the split form) for the trait-by-trait comparison")
+    ap.add_argument("--max-cols", type=int, default=2048,
+                    help="batched: widest shared K.V call (columns)")
     ap.add_argument("--quiet", action="store_true")
     ap.add_argument("--verbose", action="store_true", help="log every REML iteration to stderr")
     return ap.parse_args()
@@ -206,6 +208,9 @@ def batched(args):
     from coherit_amd.reml_model import FitConfig
     from coherit_amd.traits import SubsetStats, fit_traits, fit_traits_sequential
 
+    import logging
+
+    logging.basicConfig(level=logging.INFO, stream=sys.stderr, format="[reml] %(message)s")
     wl = synth.workload(3, scale=args.scale)
     t0 = time.perf_counter()
     st = GenoBlockStreamer(synth.SyntheticSource(wl.n, wl.p, wl.seed), call_width=131072,
@@ -223,23 +228,29 @@ def batched(args):
                     precond_rank=args.precond_rank, seed=20261203, verbose=False)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    res, info = fit_traits(st, Y, keep, X, cfg=cfg, var_components_init=init, stats=stats)
+    torch.cuda.reset_peak_memory_stats()
+    res, info = fit_traits(st, Y, keep, X, cfg=cfg, var_components_init=init, stats=stats,
+                           max_cols=args.max_cols,
+                           progress=lambda d: log(args, "batched progress", d))
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    info["peak_gib"] = round(torch.cuda.max_memory_allocated() / 2**30, 1)
+    info["panel_gib"] = round(st.panel.nbytes() / 2**30, 1)
     fph = 3600.0 * T / wall
     h2 = np.array([r.heritability for r in res])
     log(args, f"batched: {T} fits in {wall:.1f}s ({fph:.1f} fits/hour); {info['batches']} rounds, "
         f"{info['kv_sets_calls']} shared K.V calls, {info['columns']} columns, "
-        f"{info['requests']} operator requests; |h2 - true| median "
+        f"{info['requests']} operator requests, peak {info['peak_gib']} GiB; |h2 - true| median "
         f"{float(np.median(np.abs(h2 - h2_true))):.3f}")
     comp = {}
     sel = list(range(min(T, int(args.plain_traits))))
     for prec in ("fp32", "i8x2"):
         if not sel:
             break
-        c = FitConfig(**{**cfg.__dict__, "precision": prec})
+        c = FitConfig(**{**cfg.__dict__, "precision": prec, "verbose": True})
         st.set_precision(prec)
-        r_seq, inf = fit_traits_sequential(st, Y, keep, X, cfg=c, var_components_init=init[sel],
+        log(args, f"{prec} trait by trait on traits {sel} ...")
+        r_seq, inf = fit_traits_sequential(st, Y, keep, X, cfg=c, var_components_init=init,
                                            stats=stats, traits=sel)
         st.set_precision("i8x2")
         per = inf["wall_s"] / len(sel)

@@ -242,6 +242,46 @@ extern "C" int coh_kv_sets(const coh_panel* p, int32_t precision, const float* V
                 static_cast<uint8_t*>(ws), (hipStream_t)stream);
 }
 
+extern "C" int coh_xtv_sets(const coh_panel* p, int32_t precision, const float* V, int64_t k,
+                            int64_t ldv, int32_t cb, int32_t ce, int32_t nsets,
+                            const float* set_mean, const float* set_inv, int64_t set_stride,
+                            const int32_t* col_set, float scale, float* out, int64_t ldo,
+                            void* ws, uint64_t ws_bytes, void* stream) {
+  int st = validate_common(p, V, k, ldv, cb, ce);
+  if (st != COH_OK) return st;
+  COH_REQUIRE(precision == COH_PREC_I8X3 || precision == COH_PREC_I8X2, COH_ERR_VALUE,
+              "statistic sets need an int8-limb precision");
+  COH_REQUIRE(nsets >= 1, COH_ERR_VALUE, "nsets must be positive");
+  COH_REQUIRE(set_mean && set_inv && col_set, COH_ERR_VALUE, "NULL statistic-set argument");
+  COH_REQUIRE(set_stride >= p->m_slot, COH_ERR_VALUE, "set_stride %lld < m_slot %lld",
+              (long long)set_stride, (long long)p->m_slot);
+  COH_REQUIRE(out != nullptr, COH_ERR_VALUE, "NULL out");
+  COH_REQUIRE(ldo >= k, COH_ERR_VALUE, "ldo < k");
+  KvPlan pl{};
+  st = make_plan(p, precision, k, cb, ce, &pl);
+  if (st != COH_OK) return st;
+  COH_REQUIRE(ws != nullptr && ws_bytes >= (uint64_t)pl.total, COH_ERR_VALUE,
+              "workspace too small: %llu < %lld bytes", (unsigned long long)ws_bytes,
+              (long long)pl.total);
+  CompScale cs{};
+  cs.ncomp = ce - cb;
+  for (int g = 0; g < cs.ncomp; ++g) {
+    cs.end[g] = p->comp_slot_end[cb + g];
+    cs.scale[g] = scale;
+  }
+  const StatSets ss{set_mean, set_inv, set_stride, nsets, col_set, nullptr};
+  hipStream_t s = (hipStream_t)stream;
+  uint8_t* w = static_cast<uint8_t*>(ws);
+  st = run_phase1_mfma(p, pl, V, k, ldv, cs, ss, 1, w, s);
+  if (st != COH_OK) return st;
+  const float* U = reinterpret_cast<const float*>(w + pl.off_U);
+  const int64_t nslots = pl.slot1 - pl.slot0;
+  hipLaunchKernelGGL(k_copy_rows, dim3(grid_for(nslots * k)), dim3(256), 0, s, U, pl.ldu, nslots,
+                     (int)k, out, ldo);
+  COH_CHECK_LAUNCH();
+  return COH_OK;
+}
+
 extern "C" int coh_xtv(const coh_panel* p, int32_t precision, const float* V, int64_t k,
                        int64_t ldv, int32_t cb, int32_t ce, float scale, int32_t slot_space,
                        float* out, int64_t ldo, void* ws, uint64_t ws_bytes, void* stream) {

@@ -224,9 +224,12 @@ def build_lowrank_basis(K_mv: Callable, n: int, max_rank: int, key, oversample: 
         dev = torch.device("cuda", torch.cuda.current_device())
         omega = gaussian_sketch(n, r, int(key), dev)
     Y = K_mv(omega)
+    omega = None  # each n x r block is released once consumed
     Q = orthonormal_basis(Y)
+    Y = None
     Z = K_mv(Q)
     Bm = (Q.T @ Z).double()
+    Z = None
     Bm = 0.5 * (Bm + Bm.T)
     evals, evecs = torch.linalg.eigh(Bm)
     idx = torch.argsort(evals, descending=True)
@@ -237,7 +240,46 @@ def build_lowrank_basis(K_mv: Callable, n: int, max_rank: int, key, oversample: 
     return U, evals[:k].to(torch.float32)
 
 
+def build_lowrank_basis_and_cores(K_mv: Callable, gram_fn: Callable, weights, n: int,
+                                  max_rank: int, key, oversample: int = 8,
+                                  omega: Optional[torch.Tensor] = None):
+    """Nystrom basis of K = sum_g w_g K_g and its projected cores U^T K_g U
+    with one pass over the panel after the sketch.
+
+    The reference builds the basis with two passes over K (precond.py:251-276)
+    and then the cores U^T K_g U with a third (reml_model.py:914-931,
+    kv_impl.py:1287-1339).  With G_g = Q^T K_g Q from one pass over the
+    orthonormal sketch range Q (``gram_fn(Q)`` -> (G, r, r)):
+        Q^T K Q = sum_g w_g G_g,    U = Q E,    U^T K_g U = E^T G_g E
+    which are the same quantities in exact arithmetic.  Returns
+    ``(U (n, k), evals (k,), cores (G, k, k))``.
+    """
+    r = int(max_rank) + int(oversample)
+    if omega is None:
+        dev = torch.device("cuda", torch.cuda.current_device())
+        omega = gaussian_sketch(n, r, int(key), dev)
+    Y = K_mv(omega)
+    omega = None
+    Q = orthonormal_basis(Y)
+    Y = None
+    Gq = gram_fn(Q).double()
+    Gq = 0.5 * (Gq + Gq.transpose(-1, -2))
+    w = torch.as_tensor(np.asarray(weights, np.float64).reshape(-1), dtype=torch.float64,
+                        device=Gq.device)
+    Bm = (w[:, None, None] * Gq).sum(0)
+    evals, evecs = torch.linalg.eigh(Bm)
+    idx = torch.argsort(evals, descending=True)
+    evals = torch.clamp(evals[idx], min=0.0)
+    evecs = evecs[:, idx]
+    k = min(int(max_rank), int(Q.shape[1]))
+    E = evecs[:, :k]
+    U = Q @ E.to(Q.dtype)
+    cores = E.T[None] @ Gq @ E[None]
+    return U, evals[:k].to(torch.float32), cores.to(torch.float32)
+
+
 __all__ = [
+    "build_lowrank_basis_and_cores",
     "ProjectedCorePrecondConf",
     "ProjectedCoreRuntime",
     "build_projected_core_runtime",

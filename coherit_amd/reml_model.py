@@ -18,7 +18,8 @@ import torch
 
 from .geno_stream import GenoBlockStreamer
 from .pcg import pcg_solve
-from .precond import (ProjectedCorePrecondConf, build_lowrank_basis, make_precond,
+from .precond import (ProjectedCorePrecondConf, build_lowrank_basis,
+                      build_lowrank_basis_and_cores, make_precond,
                       scalar_diag_from_precond_conf)
 from .reml import REMLProbeCache, fit_reml, standardize_response
 from .slq import default_workspace_bytes
@@ -289,13 +290,19 @@ class InfinitesimalREMLFitter:
                     acc = t if acc is None else acc + t
                 return acc
         n = self.streamers[0].n
-        U, evals = build_lowrank_basis(K_ref, n, cfg.precond_rank, cfg.seed + 4321)
+        if G > 1 and ops.projected_core_atoms is not None:
+            # basis and cores from the per-component Grams of the sketch
+            # range: one panel pass fewer than basis-then-cores
+            U, evals, raw = build_lowrank_basis_and_cores(
+                K_ref, ops.projected_core_atoms, theta_ref, n, cfg.precond_rank, cfg.seed + 4321)
+        else:
+            U, evals = build_lowrank_basis(K_ref, n, cfg.precond_rank, cfg.seed + 4321)
         r = int(U.shape[1])
         eye = torch.eye(r, dtype=torch.float32, device=U.device)
         if G == 1:
             raw = torch.diag(evals.to(U.dtype))[None]
         elif ops.projected_core_atoms is not None:
-            raw = ops.projected_core_atoms(U)
+            pass
         else:
             KU = torch.stack([mv(U) for mv in ops.K_mvs])
             raw = torch.einsum("nr,gns->grs", U, KU)

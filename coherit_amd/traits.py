@@ -43,7 +43,8 @@ import torch
 from . import _native as N
 from .reml_model import FitConfig, FitResult, InfinitesimalREMLFitter
 
-__all__ = ["SubsetStats", "TraitView", "PassBatcher", "fit_traits", "fit_traits_sequential"]
+__all__ = ["SubsetStats", "TraitView", "PassBatcher", "fit_traits", "fit_traits_sequential",
+           "trait_bytes", "admissible_fits"]
 
 
 # ------------------------------------------------------------- statistics
@@ -112,12 +113,68 @@ class SubsetStats:
 # ---------------------------------------------------------------- batcher
 @dataclasses.dataclass
 class _Request:
+    """One operator call of one trait's fit.
+
+    ``kind`` says what the fit needs from the stacked products K_g V, so
+    that only that is kept: "stacked" (G, n_t, k); "component" K_g V for
+    ``comp``; "weighted" theta_e V + sum_g theta_g K_g V (``theta`` = G
+    genetic weights, ``theta_e`` or None); "cores" (G, k, k) = V^T K_g V.
+    """
     trait: int
     V: torch.Tensor            # (n_t, k) float32, trait space
     precision: str
+    kind: str = "stacked"
+    comp: int = 0
+    theta: tuple = ()
+    theta_e: float | None = None
     result: torch.Tensor | None = None
     error: BaseException | None = None
     done: bool = False
+
+    def allocate(self, G: int) -> None:
+        n_t, k = int(self.V.shape[0]), int(self.V.shape[1])
+        shape = {"stacked": (G, n_t, k), "component": (n_t, k), "weighted": (n_t, k),
+                 "cores": (G, k, k)}[self.kind]
+        self.result = torch.empty(shape, dtype=torch.float32, device=self.V.device)
+
+    def take(self, out_rows, c0: int, c1: int) -> None:
+        """Store columns [c0, c1) from ``out_rows(g)`` = (n_t, c1 - c0) of K_g V."""
+        G = self.result.shape[0] if self.kind in ("stacked", "cores") else len(self.theta)
+        if self.kind == "stacked":
+            for g in range(G):
+                self.result[g, :, c0:c1] = out_rows(g)
+        elif self.kind == "component":
+            self.result[:, c0:c1] = out_rows(self.comp)
+        elif self.kind == "weighted":
+            acc = None
+            for g, th in enumerate(self.theta):
+                if th == 0.0:
+                    continue
+                term = out_rows(g)
+                acc = term.mul_(th) if acc is None else acc.add_(term, alpha=th)
+            blk = self.result[:, c0:c1]
+            if acc is None:
+                blk.zero_()
+            else:
+                blk.copy_(acc)
+            if self.theta_e is not None:
+                blk.add_(self.V[:, c0:c1], alpha=self.theta_e)
+        else:
+            for g in range(G):
+                self.result[g, :, c0:c1] = self.V.T @ out_rows(g)
+
+    def take_grams(self, slot_rows, eff) -> None:
+        """Cores from the phase-1 product in slot space: ``slot_rows(g)`` =
+        (Z_g^T V) rows of component g (all k columns); core_g = T_g^T T_g /
+        eff_g, as the streamer's projected-core atoms (kv_impl.py:198-218)."""
+        for g in range(self.result.shape[0]):
+            e = float(eff[g])
+            if e > 0:
+                Tg = slot_rows(g)
+                torch.matmul(Tg.T, Tg, out=self.result[g])
+                self.result[g].div_(e)
+            else:
+                self.result[g].zero_()
 
 
 class PassBatcher:
@@ -129,13 +186,17 @@ class PassBatcher:
     is in.  Batches wider than ``max_cols`` columns are split.
     """
 
-    def __init__(self, streamer, stats: SubsetStats, n_fits: int, *, max_cols: int = 2048):
+    def __init__(self, streamer, stats: SubsetStats, n_fits: int, *, max_cols: int = 2048,
+                 progress=None, progress_every_s: float = 20.0):
         self.streamer = streamer
+        self.progress = progress
+        self.progress_every_s = float(progress_every_s)
+        self._t_start = self._t_report = time.perf_counter()
         self.panel = streamer.panel
         self.stats = stats
         self.max_cols = int(max_cols)
         self._cv = threading.Condition()
-        self._live = int(n_fits)
+        self._live = int(n_fits)  # fits in the rendezvous (see join / leave)
         self._pending: list[_Request] = []
         self.n_batches = 0
         self.n_calls = 0
@@ -155,6 +216,11 @@ class PassBatcher:
             raise req.error
         return req.result
 
+    def join(self) -> None:
+        """Admit one more fit to the rendezvous."""
+        with self._cv:
+            self._live += 1
+
     def leave(self) -> None:
         with self._cv:
             self._live -= 1
@@ -173,34 +239,44 @@ class PassBatcher:
                 if not r.done:
                     r.error = e
         finally:
-            self.busy_s += time.perf_counter() - t0
+            now = time.perf_counter()
+            self.busy_s += now - t0
             self.n_batches += 1
             for r in batch:
                 r.done = True
             self._cv.notify_all()
+            if self.progress is not None and now - self._t_report >= self.progress_every_s:
+                self._t_report = now
+                self.progress(dict(elapsed_s=round(now - self._t_start, 1), rounds=self.n_batches,
+                                   live=self._live, requests=self.requests, columns=self.columns,
+                                   contract_s=round(self.busy_s, 1)))
 
     def _contract(self, reqs: Sequence[_Request], prec: str) -> None:
-        # column pieces (request, c0, c1), packed greedily up to max_cols
-        pieces = []
-        for r in reqs:
-            k = int(r.V.shape[1])
-            r.result = torch.empty((self.panel.n_components, r.V.shape[0], k),
-                                   dtype=torch.float32, device=r.V.device)
-            for c0 in range(0, k, self.max_cols):
-                pieces.append((r, c0, min(k, c0 + self.max_cols)))
-        group, width = [], 0
-        for piece in pieces:
-            w = piece[2] - piece[1]
-            if group and width + w > self.max_cols:
-                self._call(group, width, prec)
-                group, width = [], 0
-            group.append(piece)
-            width += w
-        if group:
-            self._call(group, width, prec)
+        # projected-core Grams need only Z^T V (phase 1) when a request fits
+        # one call; everything else takes the stacked K.V
+        grams = [r for r in reqs if r.kind == "cores" and int(r.V.shape[1]) <= self.max_cols]
+        kvs = [r for r in reqs if not (r.kind == "cores" and int(r.V.shape[1]) <= self.max_cols)]
+        for route, rs in (("xtv", grams), ("kv", kvs)):
+            # column pieces (request, c0, c1), packed greedily up to max_cols
+            pieces = []
+            for r in rs:
+                k = int(r.V.shape[1])
+                r.allocate(self.panel.n_components)
+                for c0 in range(0, k, self.max_cols):
+                    pieces.append((r, c0, min(k, c0 + self.max_cols)))
+            group, width = [], 0
+            for piece in pieces:
+                w = piece[2] - piece[1]
+                if group and width + w > self.max_cols:
+                    self._call(group, width, prec, route)
+                    group, width = [], 0
+                group.append(piece)
+                width += w
+            if group:
+                self._call(group, width, prec, route)
         self.requests += len(reqs)
 
-    def _call(self, group, width: int, prec: str) -> None:
+    def _call(self, group, width: int, prec: str, route: str = "kv") -> None:
         panel, st = self.panel, self.stats
         dev = panel.device
         n, G = panel.n, panel.n_components
@@ -213,8 +289,29 @@ class PassBatcher:
             col_set[c:c + w] = r.trait
             c += w
         col_set_dev = torch.as_tensor(col_set, device=dev)
-        out = torch.empty((G, n, width), dtype=torch.float32, device=dev)
         precision = N.PRECISIONS[prec]
+        if route == "xtv":
+            base = int(panel.comp_slot_begin[0])
+            rows = int(panel.comp_slot_end[G - 1]) - base
+            T = torch.empty((rows, width), dtype=torch.float32, device=dev)
+            nb = panel.workspace_bytes(N.OP_XTV, precision, width, 0, G)
+            ws = torch.empty(max(nb, 256), dtype=torch.uint8, device=dev)
+            N.call("coh_xtv_sets", panel.handle, int(precision), N.ptr(V), int(width),
+                   int(V.stride(0)), 0, int(G), int(st.n_subsets), N.ptr(st.mean), N.ptr(st.inv),
+                   int(st.stride), N.ptr(col_set_dev), 1.0, N.ptr(T), int(T.stride(0)), N.ptr(ws),
+                   nb, N.stream_ptr())
+            c = 0
+            for r, c0, c1 in group:
+                w = c1 - c0
+                spans = [(int(panel.comp_slot_begin[g]) - base, int(panel.comp_slot_end[g]) - base)
+                         for g in range(G)]
+                r.take_grams(lambda g, c=c, w=w, spans=spans: T[spans[g][0]:spans[g][1], c:c + w],
+                             st.eff[r.trait])
+                c += w
+            self.n_calls += 1
+            self.columns += width
+            return
+        out = torch.empty((G, n, width), dtype=torch.float32, device=dev)
         nb = panel.workspace_bytes(N.OP_KV, precision, width, 0, G)
         ws = torch.empty(max(nb, 256), dtype=torch.uint8, device=dev)
         N.call("coh_kv_sets", panel.handle, int(precision), N.ptr(V), int(width), int(V.stride(0)),
@@ -224,7 +321,8 @@ class PassBatcher:
         c = 0
         for r, c0, c1 in group:
             w = c1 - c0
-            r.result[:, :, c0:c1] = out[:, st.index[r.trait], c:c + w]
+            idx = st.index[r.trait]
+            r.take(lambda g, c=c, w=w, idx=idx: out[g, :, c:c + w].index_select(0, idx), c0, c1)
             c += w
         self.n_calls += 1
         self.columns += width
@@ -267,52 +365,49 @@ class TraitView:
     def residual_precision(self) -> str:
         return "i8x3" if self.precision == "i8x2" else self.precision
 
-    def _stacked(self, V, precision=None):
+    def _submit(self, V, precision=None, **kind):
         V = torch.as_tensor(V, device=self.dev)
         squeeze = V.ndim == 1
         V = (V[:, None] if squeeze else V).to(torch.float32).contiguous()
         if V.shape[0] != self.n:
             raise ValueError(f"V must have {self.n} rows, got {V.shape[0]}.")
-        out = self._b.submit(_Request(self.trait, V, precision or self.precision))
-        return out, V, squeeze
+        out = self._b.submit(_Request(self.trait, V, precision or self.precision, **kind))
+        return out, squeeze
 
     def stacked_component_kv(self, V, normalize: bool = True):
         if not normalize:
             raise NotImplementedError("batched traits apply 1/eff per component.")
-        out, _, sq = self._stacked(V)
+        out, sq = self._submit(V)
         return out[:, :, 0] if sq else out
 
     def component_kv(self, V, component_idx: int, normalize: bool = True):
         if component_idx < 0 or component_idx >= self._G:
             raise IndexError(f"component_idx={component_idx} out of range for {self._G} components.")
-        out = self.stacked_component_kv(V, normalize)
-        return out[component_idx]
+        if not normalize:
+            raise NotImplementedError("batched traits apply 1/eff per component.")
+        out, sq = self._submit(V, kind="component", comp=int(component_idx))
+        return out[:, 0] if sq else out
 
     def kv(self, V, normalize: bool = True):
-        """All SNPs, 1/eff of the whole panel (geno_stream.py:2669-2679)."""
-        out, _, sq = self._stacked(V)
+        """All SNPs, 1/eff of the whole panel (geno_stream.py:2669-2679):
+        sum_g (eff_g / eff) K_g V."""
         eff_tot = float(self._eff.sum())
-        acc = sum(float(self._eff[g]) * out[g] for g in range(self._G))
-        acc = acc / eff_tot if (normalize and eff_tot > 0) else acc
-        return acc[:, 0] if sq else acc
+        w = [float(e) / eff_tot if (normalize and eff_tot > 0) else float(e) for e in self._eff]
+        out, sq = self._submit(V, kind="weighted", theta=tuple(w))
+        return out[:, 0] if sq else out
 
     def weighted_component_hv(self, theta_g, theta_e, V, precision: str | None = None):
         th = np.asarray(theta_g, np.float64).reshape(-1)
         if th.size != self._G:
             raise ValueError(f"theta_g length mismatch: expected {self._G}, got {th.size}.")
-        out, Vm, sq = self._stacked(V, precision)
-        acc = None
-        for g in range(self._G):
-            term = float(th[g]) * out[g]
-            acc = term if acc is None else acc + term
-        if theta_e is not None:
-            acc = acc + float(np.asarray(theta_e, np.float64).reshape(-1)[0]) * Vm
-        return acc[:, 0] if sq else acc
+        te = None if theta_e is None else float(np.asarray(theta_e, np.float64).reshape(-1)[0])
+        out, sq = self._submit(V, precision, kind="weighted", theta=tuple(float(x) for x in th),
+                               theta_e=te)
+        return out[:, 0] if sq else out
 
     def build_projected_core_atoms(self, U, *, subtract_identity: bool = True):
         """U^T K_g U [- I] per component (kv_impl.py:1287-1339)."""
-        out, Um, _ = self._stacked(U)
-        cores = torch.stack([Um.T @ out[g] for g in range(self._G)])
+        cores, _ = self._submit(U, kind="cores")
         cores = 0.5 * (cores + cores.transpose(-1, -2))
         if subtract_identity:
             cores = cores - torch.eye(cores.shape[-1], device=cores.device)[None]
@@ -338,6 +433,29 @@ def _slq_budget(cfg: FitConfig, n_t: int) -> float:
     return 8.0 * per_probe * max(1, int(cfg.slq_samples))
 
 
+def trait_bytes(cfg: FitConfig, n_t: int) -> int:
+    """Device memory one trait's fit holds at its peak: the SLQ basis and its
+    cotangent (S probes x (m+1) vectors each), two rank-r bases (the frozen
+    SLQ whitening reference and the current PCG one) plus the Nystrom
+    sketch blocks, and O(64) working vectors."""
+    S, m, r = int(cfg.slq_samples), int(cfg.slq_m), int(cfg.precond_rank)
+    rk = r + 8 if r > 0 else 0
+    return 4 * int(n_t) * (2 * S * (m + 1) + 4 * rk + 64)
+
+
+def admissible_fits(streamer, cfg: FitConfig, sizes, max_cols: int, *, headroom: float = 0.85) -> int:
+    """How many trait fits fit beside the resident panel in free HBM now:
+    free memory (minus one shared contraction's buffers) over the largest
+    trait's peak."""
+    panel = streamer.panel
+    free, _ = torch.cuda.mem_get_info(panel.device)
+    G = int(panel.n_components)
+    call = (panel.workspace_bytes(N.OP_KV, N.PRECISIONS["i8x3"], int(max_cols), 0, G)
+            + 4 * panel.n * int(max_cols) * (2 + G))
+    per = trait_bytes(cfg, int(np.max(sizes)))
+    return max(1, int((headroom * free - call) // per))
+
+
 def _trait_inputs(Y, keep, covar, t):
     idx = np.flatnonzero(keep[t])
     y = np.asarray(Y[idx, t] if Y.ndim == 2 else Y[t][idx], np.float32)
@@ -347,7 +465,8 @@ def _trait_inputs(Y, keep, covar, t):
 
 def fit_traits(streamer, Y, keep, covar=None, *, cfg: FitConfig | None = None,
                var_components_init=None, max_cols: int = 2048, stats: SubsetStats | None = None,
-               counters: list | None = None, keep_precond: bool = False):
+               counters: list | None = None, keep_precond: bool = False, progress=None,
+               max_active: int | None = None):
     """Fit T traits concurrently on one resident panel (shared passes).
 
     ``Y`` is (n, T) (entries outside ``keep`` are ignored), ``keep`` (T, n)
@@ -355,7 +474,10 @@ def fit_traits(streamer, Y, keep, covar=None, *, cfg: FitConfig | None = None,
     samples).  ``var_components_init`` is None, one vector for all traits, or
     (T, G+1).  Returns ``(results, info)`` with one ``FitResult`` per trait;
     with ``keep_precond`` ``info["precond_U"][t]`` is trait t's SLQ whitening
-    basis.
+    basis.  ``progress`` (optional) is called with a dict of batcher counters
+    about every 20 s.  At most ``max_active`` fits run at once (default: as
+    many as free HBM holds, ``admissible_fits``); the others are admitted as
+    fits finish.
     """
     cfg = cfg or FitConfig()
     keep = np.asarray(keep, dtype=bool)
@@ -363,13 +485,19 @@ def fit_traits(streamer, Y, keep, covar=None, *, cfg: FitConfig | None = None,
     T = int(keep.shape[0])
     if stats is None:
         stats = SubsetStats(streamer.panel, keep)
-    batcher = PassBatcher(streamer, stats, T, max_cols=max_cols)
+    if max_active is None:
+        max_active = admissible_fits(streamer, cfg, stats.sizes, max_cols)
+    max_active = max(1, min(T, int(max_active)))
+    batcher = PassBatcher(streamer, stats, 0, max_cols=max_cols, progress=progress)
+    gate = threading.Semaphore(max_active)
     results: list = [None] * T
     errors: list = [None] * T
     bases: list = [None] * T
     init = None if var_components_init is None else np.asarray(var_components_init, np.float64)
 
     def run(t):
+        gate.acquire()
+        batcher.join()
         try:
             view = TraitView(batcher, t, cfg.precision)
             c = dataclasses.replace(cfg, gpu_budget_bytes=_slq_budget(cfg, view.n), verbose=False)
@@ -384,6 +512,7 @@ def fit_traits(streamer, Y, keep, covar=None, *, cfg: FitConfig | None = None,
             errors[t] = e
         finally:
             batcher.leave()
+            gate.release()
 
     t0 = time.perf_counter()
     threads = [threading.Thread(target=run, args=(t,), name=f"trait-{t}") for t in range(T)]
@@ -399,7 +528,8 @@ def fit_traits(streamer, Y, keep, covar=None, *, cfg: FitConfig | None = None,
             except Exception:
                 raise e
             raise named from e
-    info = dict(wall_s=wall, batches=batcher.n_batches, kv_sets_calls=batcher.n_calls,
+    info = dict(wall_s=wall, max_active=max_active, batches=batcher.n_batches,
+                kv_sets_calls=batcher.n_calls,
                 columns=batcher.columns, requests=batcher.requests, contract_s=batcher.busy_s)
     if keep_precond:
         info["precond_U"] = bases
@@ -412,8 +542,9 @@ def fit_traits_sequential(streamer, Y, keep, covar=None, *, cfg: FitConfig | Non
     """The reference's way: one fit per trait, each on the panel standardized
     with the trait's statistics and restricted to its samples.  Runs any
     precision (the plain FP32 drop-in included); restores the panel's own
-    statistics afterwards.  Returns ``(results, info)`` like ``fit_traits``
-    (``info["wall_s"]`` per trait in ``info["trait_s"]``)."""
+    statistics afterwards.  ``var_components_init`` is None, one vector, or
+    (T, G+1) indexed by trait.  Returns ``(results, info)`` like
+    ``fit_traits`` (seconds per fitted trait in ``info["trait_s"]``)."""
     cfg = cfg or FitConfig()
     keep = np.asarray(keep, dtype=bool)
     Y = np.asarray(Y)
@@ -431,7 +562,7 @@ def fit_traits_sequential(streamer, Y, keep, covar=None, *, cfg: FitConfig | Non
             mean_c, inv_c = stats.cache_order(panel, t)
             panel.set_standardization(mean_c, inv_c)
             view = _SequentialTraitView(streamer, stats, t, cfg.precision)
-            c = dataclasses.replace(cfg, gpu_budget_bytes=_slq_budget(cfg, view.n), verbose=False)
+            c = dataclasses.replace(cfg, gpu_budget_bytes=_slq_budget(cfg, view.n))
             fitter = InfinitesimalREMLFitter(c, streamers=[view])
             y, X = _trait_inputs(Y, keep, covar, t)
             vc0 = None if init is None else (init[t] if init.ndim == 2 else init)
@@ -464,7 +595,7 @@ class _SequentialTraitView(TraitView):
         self._eff = stats.eff[trait].astype(np.float64)
         self._atoms = stats.atoms[trait]
 
-    def _stacked(self, V, precision=None):
+    def _submit(self, V, precision=None, **kind):
         V = torch.as_tensor(V, device=self.dev)
         squeeze = V.ndim == 1
         V = (V[:, None] if squeeze else V).to(torch.float32).contiguous()
@@ -474,10 +605,20 @@ class _SequentialTraitView(TraitView):
         idx = self._stats.index[self.trait]
         full = torch.zeros((panel.n, V.shape[1]), dtype=torch.float32, device=self.dev)
         full[idx] = V
+        prec = N.PRECISIONS[precision or self.precision]
+        req = _Request(self.trait, V, precision or self.precision, **kind)
+        req.allocate(self._G)
+        if req.kind == "cores":
+            T = panel.xtv(full, 0, self._G, slot_space=True, precision=prec)
+            base = int(panel.comp_slot_begin[0])
+            spans = [(int(panel.comp_slot_begin[g]) - base, int(panel.comp_slot_end[g]) - base)
+                     for g in range(self._G)]
+            req.take_grams(lambda g: T[spans[g][0]:spans[g][1]], self._eff)
+            return req.result, squeeze
         scale = np.where(self._eff > 0, 1.0 / np.maximum(self._eff, 1e-30), 0.0).astype(np.float32)
-        out = panel.kv(full, scale, 0, self._G, stacked=True,
-                       precision=N.PRECISIONS[precision or self.precision])
-        return out[:, idx], V, squeeze
+        out = panel.kv(full, scale, 0, self._G, stacked=True, precision=prec)
+        req.take(lambda g: out[g].index_select(0, idx), 0, int(V.shape[1]))
+        return req.result, squeeze
 
     def diag(self):
         return torch.tensor(float(self._stats.atom_total[self.trait]), dtype=torch.float32,

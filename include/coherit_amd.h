@@ -175,6 +175,18 @@ int coh_kv_sets(const coh_panel* panel, int32_t precision, const float* dev_V, i
                 float* dev_out, int64_t ldo, int64_t out_stride, float beta,
                 void* dev_workspace, uint64_t workspace_bytes, void* stream);
 
+/* out = scale * Z^T V in slot space ((m_slot of the requested components) x k,
+ * padding rows zero) with one statistic set per column, as in coh_kv_sets:
+ * the phase-1 half of that operator, for the projected-core Grams
+ * (Z_g^T U)^T (Z_g^T U) of many traits' bases in one pass (kv_impl.py:1287-1339
+ * per trait).  No counterpart in the reference; int8-limb precisions only.
+ * Workspace: coh_workspace_bytes(COH_OP_XTV). */
+int coh_xtv_sets(const coh_panel* panel, int32_t precision, const float* dev_V, int64_t k,
+                 int64_t ldv, int32_t comp_begin, int32_t comp_end, int32_t nsets,
+                 const float* dev_set_mean, const float* dev_set_inv, int64_t set_stride,
+                 const int32_t* dev_col_set, float scale, float* dev_out, int64_t ldo,
+                 void* dev_workspace, uint64_t workspace_bytes, void* stream);
+
 /* out = scale * Z^T V for components [comp_begin, comp_end)
  * (kv_impl.py:163-185, 382-439).  slot_space == 0: rows in cache order
  * (m, k), covering the requested components' SNPs; slot_space == 1: rows in

@@ -111,6 +111,13 @@ def test_statistic_set_operator_matches_each_subset(precision, tol):
         d = np.linalg.norm(got - ref) / np.linalg.norm(ref)
         print(precision, "trait", t, "batched vs solo", d, flush=True)
         assert d <= 1e-6
+        # projected-core Grams through the phase-1 statistic-set call
+        cq = _Request(t, r.V, precision, kind="cores")
+        batcher._contract([cq], precision)
+        core_solo = solo.build_projected_core_atoms(r.V, subtract_identity=False)
+        dc = float((cq.result - core_solo).norm() / core_solo.norm())
+        print(precision, "trait", t, "cores batched vs solo", dc, flush=True)
+        assert dc <= 1e-6
         solo.close()
     st.close()
 

@@ -96,3 +96,36 @@ def test_cholesky_qr2_basis_is_orthonormal_with_the_same_range():
     Yd = Y.double()
     resid = Yd - Q @ (Q.T @ Yd)
     assert float(resid.norm() / Yd.norm()) < 1e-6
+
+
+def test_basis_and_cores_from_grams_equal_basis_then_cores():
+    """One pass of per-component Grams Q^T K_g Q gives the same basis,
+    eigenvalues and cores U^T K_g U as the basis-then-cores route."""
+    import torch
+
+    from coherit_amd.precond import build_lowrank_basis, build_lowrank_basis_and_cores
+
+    rng = np.random.default_rng(21)
+    n, G, rank = 120, 3, 20
+    Ks = []
+    for g in range(G):
+        Z = rng.standard_normal((n, 40 + 10 * g))
+        Ks.append(torch.as_tensor(Z @ Z.T / Z.shape[1]))
+    w = np.array([0.5, 0.3, 0.2])
+    K = sum(float(w[g]) * Ks[g] for g in range(G))
+    omega = torch.as_tensor(rng.standard_normal((n, rank + 8)))
+    # the bases come back in FP32 (as on the device); products in FP64
+    Kmv = lambda V: (K @ V.double()).to(V.dtype)
+    U1, ev1 = build_lowrank_basis(Kmv, n, rank, 0, omega=omega.clone())
+    U1 = U1.double()
+    cores1 = torch.stack([U1.T @ Kg @ U1 for Kg in Ks])
+    U2, ev2, cores2 = build_lowrank_basis_and_cores(
+        Kmv, lambda Q: torch.stack([Q.double().T @ Kg @ Q.double() for Kg in Ks]), w, n, rank, 0,
+        omega=omega.clone())
+    np.testing.assert_allclose(ev2.numpy(), ev1.numpy(), rtol=1e-5, atol=1e-6)
+    # eigenvectors are defined up to sign
+    U2 = U2.double()
+    sign = torch.sign((U1 * U2).sum(0))
+    np.testing.assert_allclose((U2 * sign).numpy(), U1.numpy(), atol=1e-5)
+    S = sign[None, :, None] * sign[None, None, :]
+    np.testing.assert_allclose((cores2.double() * S).numpy(), cores1.numpy(), atol=1e-5)

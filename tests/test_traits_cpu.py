@@ -76,3 +76,84 @@ def test_slq_budget_runs_the_whole_probe_block_in_one_batch(n, S, m):
     cfg = FitConfig(slq_samples=S, slq_m=m)
     ws = slq.default_workspace_bytes(_slq_budget(cfg, n))
     assert slq.workspace_layout(n, S, m, ws)["batch_width"] == S
+
+
+@pytest.mark.parametrize("kind", ["stacked", "component", "weighted", "weighted_noe", "cores"])
+def test_requests_keep_what_the_fit_needs_from_stacked_products(kind):
+    rng = np.random.default_rng(3)
+    n, k, G = 50, 7, 3
+    Ks = [torch.as_tensor(rng.standard_normal((n, n))) for _ in range(G)]
+    Ks = [K @ K.T / n for K in Ks]
+    V = torch.as_tensor(rng.standard_normal((n, k)))
+    theta, te = (0.7, 0.0, 1.3), 0.4
+    kw = {"stacked": {}, "component": {"kind": "component", "comp": 2},
+          "weighted": {"kind": "weighted", "theta": theta, "theta_e": te},
+          "weighted_noe": {"kind": "weighted", "theta": theta},
+          "cores": {"kind": "cores"}}[kind]
+    req = _Request(0, V, "i8x2", **kw)
+    req.allocate(G)
+    req.result = req.result.to(torch.float64)
+    for c0, c1 in ((0, 3), (3, 7)):  # two column pieces
+        req.take(lambda g, c0=c0, c1=c1: (Ks[g] @ V[:, c0:c1]).clone(), c0, c1)
+    KV = torch.stack([K @ V for K in Ks])
+    want = {"stacked": KV, "component": KV[2],
+            "weighted": te * V + sum(t * KV[g] for g, t in enumerate(theta)),
+            "weighted_noe": sum(t * KV[g] for g, t in enumerate(theta)),
+            "cores": torch.stack([V.T @ KV[g] for g in range(G)])}[kind]
+    torch.testing.assert_close(req.result, want, rtol=1e-12, atol=1e-12)
+
+
+def test_admitted_fits_join_and_leave_the_rendezvous():
+    """At most two fits in flight: batches never hold more than two requests
+    and every fit completes (the others are admitted as fits finish)."""
+    log, out, errors = [], [], []
+    b = _batcher(0, log)
+    gate = threading.Semaphore(2)
+
+    def admitted(t):
+        gate.acquire()
+        b.join()
+        try:
+            _fit_body(b, t, 2 + t % 3, out)
+        finally:
+            b.leave()
+            gate.release()
+
+    th = [threading.Thread(target=admitted, args=(t,)) for t in range(6)]
+    for x in th:
+        x.start()
+    for x in th:
+        x.join(timeout=30)
+    assert max(len(batch) for batch in log) <= 2
+    assert len(out) == sum(2 + t % 3 for t in range(6))
+    for t, c, res in out:
+        assert torch.equal(res[0], torch.full((4, 1), 2.0 * (10 * t + c)))
+
+
+def _fit_body(b, t, calls, out):
+    for c in range(calls):
+        time.sleep(0.001 * ((t * 7 + c * 3) % 5))
+        V = torch.full((4, 1), float(10 * t + c))
+        out.append((t, c, b.submit(_Request(t, V, "i8x2"))))
+
+
+def test_trait_memory_estimate_counts_the_slq_and_nystrom_blocks():
+    from coherit_amd.traits import trait_bytes
+    cfg = FitConfig(slq_samples=16, slq_m=40, precond_rank=500)
+    assert trait_bytes(cfg, 1000) == 4 * 1000 * (2 * 16 * 41 + 4 * 508 + 64)
+    cfg0 = FitConfig(slq_samples=16, slq_m=40, precond_rank=0)
+    assert trait_bytes(cfg0, 1000) == 4 * 1000 * (2 * 16 * 41 + 64)
+
+
+def test_gram_requests_divide_each_component_by_its_eff():
+    rng = np.random.default_rng(4)
+    k, G = 5, 3
+    T = [torch.as_tensor(rng.standard_normal((30 + 7 * g, k))) for g in range(G)]
+    eff = np.array([12.0, 0.0, 30.0])
+    req = _Request(0, torch.zeros((9, k), dtype=torch.float64), "i8x2", kind="cores")
+    req.allocate(G)
+    req.result = req.result.to(torch.float64)
+    req.take_grams(lambda g: T[g], eff)
+    for g in range(G):
+        want = T[g].T @ T[g] / eff[g] if eff[g] > 0 else torch.zeros((k, k), dtype=torch.float64)
+        torch.testing.assert_close(req.result[g], want, rtol=1e-12, atol=1e-12)
