@@ -67,8 +67,8 @@ SIGNATURES = {
     "coh_kv": [_p, _i32, _p, _i64, _i64, _i32, _i32, _p, _i32, _p, _i64, _i64, _f32, _p, _u64, _p],
     "coh_xtv_sets": [_p, _i32, _p, _i64, _i64, _i32, _i32, _i32, _p, _p, _i64, _p, _f32, _p, _i64,
                      _p, _u64, _p],
-    "coh_kv_sets": [_p, _i32, _p, _i64, _i64, _i32, _i32, _i32, _p, _p, _i64, _p, _p, _i32, _p, _i64,
-                    _i64, _f32, _p, _u64, _p],
+    "coh_kv_sets": [_p, _i32, _p, _i64, _i64, _i32, _i32, _i32, _p, _p, _i64, _p, _p, _p, _i32,
+                    _i32, _p, _i64, _i64, _f32, _p, _u64, _p],
     "coh_xtv": [_p, _i32, _p, _i64, _i64, _i32, _i32, _f32, _i32, _p, _i64, _p, _u64, _p],
     "coh_zb": [_p, _i32, _p, _i64, _i64, _i32, _i32, _i32, _p, _i64, _i64, _f32, _p, _u64, _p],
     "coh_exact_diag": [_p, _f64, _p, _p],

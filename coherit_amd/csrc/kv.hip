@@ -207,7 +207,8 @@ extern "C" int coh_kv(const coh_panel* p, int32_t precision, const float* V, int
 extern "C" int coh_kv_sets(const coh_panel* p, int32_t precision, const float* V, int64_t k,
                            int64_t ldv, int32_t cb, int32_t ce, int32_t nsets,
                            const float* set_mean, const float* set_inv, int64_t set_stride,
-                           const int32_t* col_set, const float* set_comp_scale, int32_t stacked,
+                           const int32_t* col_set, const float* set_comp_scale,
+                           const int32_t* set_comp_map, int32_t nout, int32_t stacked,
                            float* out, int64_t ldo, int64_t out_stride, float beta, void* ws,
                            uint64_t ws_bytes, void* stream) {
   int st = validate_common(p, V, k, ldv, cb, ce);
@@ -223,7 +224,10 @@ extern "C" int coh_kv_sets(const coh_panel* p, int32_t precision, const float* V
   COH_REQUIRE(out != nullptr, COH_ERR_VALUE, "NULL out");
   COH_REQUIRE(ldo >= k, COH_ERR_VALUE, "ldo < k");
   COH_REQUIRE(beta == 0.0f || beta == 1.0f, COH_ERR_VALUE, "beta must be 0 or 1");
-  COH_REQUIRE(!stacked || out_stride >= p->n * ldo || ce - cb == 1, COH_ERR_VALUE,
+  const int nstack = set_comp_map ? nout : ce - cb;
+  COH_REQUIRE(!set_comp_map || (nout >= 1 && nout <= ce - cb), COH_ERR_VALUE,
+              "nout must be in [1, %d] with a component map", ce - cb);
+  COH_REQUIRE(!stacked || out_stride >= p->n * ldo || nstack == 1, COH_ERR_VALUE,
               "stacked outputs overlap (out_stride too small)");
   KvPlan pl{};
   st = make_plan(p, precision, k, cb, ce, &pl);
@@ -237,7 +241,9 @@ extern "C" int coh_kv_sets(const coh_panel* p, int32_t precision, const float* V
     cs.end[g] = p->comp_slot_end[cb + g];
     cs.scale[g] = 0.0f;  // the per-set factors apply
   }
-  const StatSets ss{set_mean, set_inv, set_stride, nsets, col_set, set_comp_scale};
+  StatSets ss{set_mean, set_inv, set_stride, nsets, col_set, set_comp_scale};
+  ss.comp_map = set_comp_map;
+  ss.nout = set_comp_map ? nout : 0;
   return run_kv(p, pl, precision, V, k, ldv, cs, ss, stacked, out, ldo, out_stride, beta,
                 static_cast<uint8_t*>(ws), (hipStream_t)stream);
 }

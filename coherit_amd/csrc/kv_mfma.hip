@@ -670,7 +670,7 @@ __global__ __launch_bounds__(256) void k_p2_mfma(P2Args a) {
 template <int L>
 __global__ void k_p2_reduce(const int32_t* __restrict__ part, int64_t n, int64_t k,
                             const double* __restrict__ scale, const double* __restrict__ wsum,
-                            ChunkTable tab, int nout, int stacked,
+                            ChunkTable tab, int nout, int stacked, StatSets ss,
                             float* __restrict__ out, int64_t ldo, int64_t out_stride, float beta) {
   const int64_t total = (int64_t)nout * n * k;
   for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
@@ -678,10 +678,14 @@ __global__ void k_p2_reduce(const int32_t* __restrict__ part, int64_t n, int64_t
     const int64_t col = idx % k;
     const int64_t i = (idx / k) % n;
     const int go = static_cast<int>(idx / (k * n));
-    const int g0 = stacked ? go : 0;
-    const int g1 = stacked ? go + 1 : tab.ncomp;
+    // stacked outputs: one per component, or the column's set's grouping
+    const int32_t* cmap =
+        (stacked && ss.comp_map) ? ss.comp_map + (int64_t)ss.set_of(col) * tab.ncomp : nullptr;
+    const int g0 = (stacked && !cmap) ? go : 0;
+    const int g1 = (stacked && !cmap) ? go + 1 : tab.ncomp;
     double acc = 0.0;
     for (int g = g0; g < g1; ++g) {
+      if (cmap && cmap[g] != go) continue;
       int64_t s[3] = {0, 0, 0};
       for (int ch = tab.chunk_off[g]; ch < tab.chunk_off[g + 1]; ++ch) {
         const int32_t* src = part + (((int64_t)ch * n + i) * k + col) * L;
@@ -976,9 +980,9 @@ static int phase2(const coh_panel* p, const KvPlan& pl, int64_t k, const StatSet
   if (!launched) return fail(COH_ERR_VALUE, "no compiled phase-2 kernel for this MFMA shape");
   tm.stop();
   COH_CHECK_LAUNCH();
-  const int nout = stacked ? pl.tab.ncomp : 1;
+  const int nout = stacked ? (ss.comp_map ? ss.nout : pl.tab.ncomp) : 1;
   hipLaunchKernelGGL(k_p2_reduce<L>, dim3(grid1d((int64_t)nout * p->n * k)), dim3(256), 0, st,
-                     a.part, p->n, k, scale, sv.wsum, pl.tab, nout, stacked, out, ldo,
+                     a.part, p->n, k, scale, sv.wsum, pl.tab, nout, stacked, ss, out, ldo,
                      out_stride, beta);
   COH_CHECK_LAUNCH();
   return COH_OK;

@@ -30,7 +30,9 @@ struct CompScale {
 // set per sample subset with a set index per right-hand column: column c
 // uses mean/inv[set * stride + slot] and component factor
 // comp_scale[set * ncomp + g] (device arrays).  Set indices are clamped to
-// [0, nsets).
+// [0, nsets).  With comp_map, a set also groups the call's panel components
+// into its own output components: stacked output o of column c sums the
+// panel components g with comp_map[set * ncomp + g] == o (-1: none).
 struct StatSets {
   const float* mean;
   const float* inv;
@@ -38,6 +40,8 @@ struct StatSets {
   int nsets;
   const int32_t* col_set;
   const float* comp_scale;
+  const int32_t* comp_map = nullptr;
+  int nout = 0;  // stacked outputs per column with comp_map
   __device__ int set_of(int64_t col) const {
     if (col_set == nullptr) return 0;
     const int t = col_set[col];

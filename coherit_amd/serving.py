@@ -1,0 +1,181 @@
+"""Serving: REML fits of arriving requests on one resident panel.
+
+Each request is a fit on its own sample subset with its own partition of
+the SNPs into 1..8 contiguous components (the panel is built from
+contiguous blocks and a request's components are runs of blocks), with its
+own covariates and phenotype.  The reference answers each such request with
+a separate streamer (``sample_mask``, geno_stream.py:1161-1247) and
+``fit_infinitesimal`` (reml_model.py:1836-2010); here every request is that
+same fit, and requests that are on the GPU at the same time share every
+pass over the panel:
+
+* admission: a request waits (in arrival order) until a statistic set is
+  free and its peak device memory (``traits.trait_bytes``) fits in the HBM
+  budget left by the requests already admitted; it is admitted as soon as
+  earlier fits finish and free both;
+* each admitted request takes a row of a ``StatSetPool`` (its subset
+  statistics, block -> component map, eff and atoms) and runs the fitter on
+  a ``TraitView`` in its own host thread; its solver state stays on the
+  device for the whole fit;
+* the ``PassBatcher`` packs the operator calls of all admitted fits into
+  shared ``coh_kv_sets`` / ``coh_xtv_sets`` contractions whenever every
+  admitted fit is waiting, so requests that arrive while others run join
+  the next pass (continuous batching).
+
+A request's result does not depend on the mix: its columns carry their own
+statistic set, per-column quantisation scales and exact integer partial
+sums, so it is the same alone and among other requests (tested).
+"""
+from __future__ import annotations
+
+import dataclasses
+import threading
+import time
+from typing import Sequence
+
+import numpy as np
+import torch
+
+from .reml_model import FitConfig, FitResult, InfinitesimalREMLFitter
+from .traits import PassBatcher, StatSetPool, TraitView, _slq_budget, trait_bytes
+
+__all__ = ["FitRequest", "ServedFit", "FitServer"]
+
+
+@dataclasses.dataclass
+class FitRequest:
+    """One REML fit: samples ``keep`` (n,) bool, components ``groups`` (runs
+    of consecutive panel blocks), phenotype ``y`` (n,) and covariates ``X``
+    (n, p) (rows outside ``keep`` ignored), arrival time in seconds from the
+    start of the run."""
+    keep: np.ndarray
+    groups: list
+    y: np.ndarray
+    X: np.ndarray | None = None
+    arrival_s: float = 0.0
+    var_components_init: np.ndarray | None = None
+
+
+@dataclasses.dataclass
+class ServedFit:
+    request: int
+    result: FitResult | None
+    error: BaseException | None
+    arrival_s: float
+    admitted_s: float
+    done_s: float
+    n_samples: int
+    n_components: int
+    precond_U: torch.Tensor | None = None
+
+    @property
+    def latency_s(self) -> float:
+        return self.done_s - self.arrival_s
+
+    @property
+    def queue_s(self) -> float:
+        return self.admitted_s - self.arrival_s
+
+
+class FitServer:
+    """Continuous-batching REML fits on one resident panel (``streamer``)."""
+
+    def __init__(self, streamer, cfg: FitConfig | None = None, *, capacity: int = 64,
+                 max_cols: int = 1024, hbm_budget_bytes: float | None = None,
+                 headroom: float = 0.85):
+        self.streamer = streamer
+        self.cfg = cfg or FitConfig(precision="i8x2")
+        if self.cfg.precision not in ("i8x2", "i8x3"):
+            raise ValueError("the server runs the int8-limb forms ('i8x2' or 'i8x3').")
+        self.pool = StatSetPool(streamer.panel, capacity)
+        self.max_cols = int(max_cols)
+        panel = streamer.panel
+        if hbm_budget_bytes is None:
+            free, _ = torch.cuda.mem_get_info(panel.device)
+            G = int(panel.n_components)
+            from . import _native as N
+            call = (panel.workspace_bytes(N.OP_KV, N.PRECISIONS["i8x3"], self.max_cols, 0, G)
+                    + 4 * panel.n * self.max_cols * (2 + 8))
+            hbm_budget_bytes = headroom * free - call
+        self.budget = float(hbm_budget_bytes)
+
+    def run(self, requests: Sequence[FitRequest], *, progress=None,
+            keep_precond: bool = False) -> list[ServedFit]:
+        """Serve ``requests`` (arrivals relative to now); returns one
+        ``ServedFit`` per request, in request order (with each fit's SLQ
+        whitening basis when ``keep_precond``)."""
+        cfg = self.cfg
+        batcher = PassBatcher(self.streamer, self.pool, 0, max_cols=self.max_cols,
+                              progress=progress)
+        order = sorted(range(len(requests)), key=lambda i: requests[i].arrival_s)
+        rank = {i: q for q, i in enumerate(order)}
+        gate = threading.Condition()
+        state = {"next": 0, "used": 0.0}
+        out: list = [None] * len(requests)
+        t0 = time.perf_counter()
+        now = lambda: time.perf_counter() - t0
+
+        def serve(i):
+            req = requests[i]
+            delay = req.arrival_s - now()
+            if delay > 0:
+                time.sleep(delay)
+            n_t = int(np.count_nonzero(req.keep))
+            need = float(trait_bytes(cfg, n_t))
+            # admission in arrival order, by free statistic set and HBM budget
+            with gate:
+                while not (state["next"] == rank[i] and self.pool.free_rows() > 0
+                           and (state["used"] + need <= self.budget or state["used"] == 0.0)):
+                    gate.wait()
+                state["next"] += 1
+                state["used"] += need
+                gate.notify_all()
+            admitted = now()
+            row = None
+            res = err = basis = None
+            batcher.join()
+            try:
+                row = self.pool.add(req.keep, req.groups)
+                view = TraitView(batcher, row, cfg.precision)
+                c = dataclasses.replace(cfg, gpu_budget_bytes=_slq_budget(cfg, n_t), verbose=False)
+                fitter = InfinitesimalREMLFitter(c, streamers=[view])
+                idx = np.flatnonzero(req.keep)
+                y = np.asarray(req.y, np.float32)[idx]
+                X = None if req.X is None else np.asarray(req.X, np.float32)[idx]
+                res = fitter.fit_infinitesimal(y, X, var_components_init=req.var_components_init)
+                if keep_precond and fitter.slq_precond_conf is not None:
+                    basis = fitter.slq_precond_conf.U
+            except BaseException as e:
+                err = e
+            finally:
+                batcher.leave()
+                if row is not None:
+                    self.pool.release(row)
+                with gate:
+                    state["used"] -= need
+                    gate.notify_all()
+            out[i] = ServedFit(i, res, err, req.arrival_s, admitted, now(), n_t, len(req.groups),
+                               basis)
+
+        threads = [threading.Thread(target=serve, args=(i,), name=f"request-{i}")
+                   for i in range(len(requests))]
+        for th in threads:
+            th.start()
+        for th in threads:
+            th.join()
+        self.last_batcher = batcher
+        return out
+
+
+def latency_summary(fits: Sequence[ServedFit]) -> dict:
+    """p50 / p99 latency (s) and completed fits per hour over the run."""
+    ok = [f for f in fits if f.error is None]
+    lat = np.array([f.latency_s for f in ok])
+    if lat.size == 0:
+        return dict(completed=0)
+    span = max(f.done_s for f in ok) - min(f.arrival_s for f in ok)
+    return dict(completed=len(ok), failed=len(fits) - len(ok),
+                p50_latency_s=float(np.percentile(lat, 50)),
+                p99_latency_s=float(np.percentile(lat, 99)),
+                mean_queue_s=float(np.mean([f.queue_s for f in ok])),
+                fits_per_hour=float(3600.0 * len(ok) / max(span, 1e-9)))

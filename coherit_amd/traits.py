@@ -103,11 +103,124 @@ class SubsetStats:
         inv_eff = np.where(self.eff > 0, 1.0 / np.maximum(self.eff, 1e-30), 0.0).astype(np.float32)
         self.comp_scale = torch.as_tensor(inv_eff, device=dev).contiguous()
         self.index = [torch.as_tensor(np.flatnonzero(keep[t]), device=dev) for t in range(T)]
+        # every set uses the panel's own components
+        self.comp_map = None
+        self.nout = [G] * T
+        base = int(panel.comp_slot_begin[0])
+        spans = [(int(panel.comp_slot_begin[g]) - base, int(panel.comp_slot_end[g]) - base)
+                 for g in range(G)]
+        self.spans = [spans] * T
 
     def cache_order(self, panel, t: int):
         """Trait t's (mean, inv_sd) in cache order (m,), for a panel override."""
         idx = torch.as_tensor(panel.cache_to_slot_host.astype(np.int64), device=self.mean.device)
         return self.mean[t].index_select(0, idx).contiguous(), self.inv[t].index_select(0, idx).contiguous()
+
+
+class StatSetPool:
+    """Statistic sets that come and go: the serving form.
+
+    ``capacity`` rows of slot-space mean/inv; each admitted request takes a
+    row with its sample subset's statistics (``coh_panel_subset_stats``) and
+    its own partition of the SNPs into components, given as contiguous runs
+    of the panel's components (blocks).  Per row: the block -> request
+    component map, the per-block factor 1/eff of the block's component, and
+    eff / trace atoms per request component (geno_stream.py:1394-1448 over
+    the request's samples and components).
+    """
+
+    def __init__(self, panel, capacity: int):
+        self.panel = panel
+        cap = int(capacity)
+        dev = panel.device
+        self.n_subsets = cap
+        self.stride = int(panel.m_slot)
+        Gp = int(panel.n_components)
+        self.mean = torch.zeros((cap, self.stride), dtype=torch.float32, device=dev)
+        self.inv = torch.zeros((cap, self.stride), dtype=torch.float32, device=dev)
+        self.comp_scale = torch.zeros((cap, Gp), dtype=torch.float32, device=dev)
+        self.comp_map = torch.full((cap, Gp), -1, dtype=torch.int32, device=dev)
+        self.index = [None] * cap
+        self.sizes = np.zeros(cap, np.int64)
+        self.eff = [None] * cap
+        self.atoms = [None] * cap
+        self.atom_total = np.zeros(cap)
+        self.nout = [0] * cap
+        self.spans = [None] * cap
+        self._free = list(range(cap))[::-1]
+        self._lock = threading.Lock()
+        self._cnt = torch.empty((1, self.stride), dtype=torch.int64, device=dev)
+        nb = ctypes.c_uint64()
+        N.call("coh_subset_stats_workspace_bytes", panel.handle, 1, ctypes.byref(nb))
+        self._ws = torch.empty(max(int(nb.value), 256), dtype=torch.uint8, device=dev)
+        self._ws_bytes = int(nb.value)
+        self._base = int(panel.comp_slot_begin[0])
+
+    def free_rows(self) -> int:
+        with self._lock:
+            return len(self._free)
+
+    def add(self, keep, groups) -> int:
+        """Take a row for samples ``keep`` (n,) bool and components ``groups``
+        (list of lists of consecutive panel block indices)."""
+        panel = self.panel
+        keep = np.asarray(keep, dtype=bool).reshape(-1)
+        if keep.size != panel.n or not keep.any():
+            raise ValueError(f"keep must select samples out of {panel.n}.")
+        Gp = int(panel.n_components)
+        cmap = np.full(Gp, -1, np.int32)
+        for g, blocks in enumerate(groups):
+            blocks = [int(b) for b in blocks]
+            if not blocks or blocks != list(range(blocks[0], blocks[0] + len(blocks))):
+                raise ValueError("each component must be a run of consecutive panel blocks.")
+            if min(blocks) < 0 or max(blocks) >= Gp or np.any(cmap[blocks] >= 0):
+                raise ValueError("component blocks out of range or overlapping.")
+            cmap[blocks] = g
+        with self._lock:
+            if not self._free:
+                raise RuntimeError("no free statistic set (capacity reached).")
+            row = self._free.pop()
+        dev = panel.device
+        # the subset stats run on the calling thread's stream, in order with
+        # the request's later operator calls
+        with self._lock:
+            keep_dev = torch.as_tensor(keep.astype(np.uint8), device=dev)
+            N.call("coh_panel_subset_stats", panel.handle, 1, N.ptr(keep_dev), N.ptr(self.mean[row]),
+                   N.ptr(self.inv[row]), N.ptr(self._cnt), self.stride, N.ptr(self._ws),
+                   self._ws_bytes, N.stream_ptr())
+            valid = self.inv[row] > 0.0
+            cntf = torch.where(valid, self._cnt[0], torch.zeros_like(self._cnt[0])).double()
+            effb = torch.stack([valid[int(panel.comp_slot_begin[b]):int(panel.comp_slot_end[b])].sum()
+                                for b in range(Gp)]).double().cpu().numpy()
+            massb = torch.stack([cntf[int(panel.comp_slot_begin[b]):int(panel.comp_slot_end[b])].sum()
+                                 for b in range(Gp)]).cpu().numpy()
+        G = len(groups)
+        n_t = float(keep.sum())
+        eff = np.array([effb[g_blocks].sum() for g_blocks in groups], np.float64)
+        mass = np.array([massb[g_blocks].sum() for g_blocks in groups], np.float64)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            atoms = np.where(eff > 0, mass / (n_t * eff), 0.0)
+        factor = np.zeros(Gp, np.float32)
+        for g, g_blocks in enumerate(groups):
+            if eff[g] > 0:
+                factor[g_blocks] = np.float32(1.0 / np.float32(eff[g]))
+        self.comp_scale[row] = torch.as_tensor(factor, device=dev)
+        self.comp_map[row] = torch.as_tensor(cmap, device=dev)
+        self.index[row] = torch.as_tensor(np.flatnonzero(keep), device=dev)
+        self.sizes[row] = int(n_t)
+        self.eff[row] = eff.astype(np.float32)
+        self.atoms[row] = atoms.astype(np.float32)
+        tot = eff.sum()
+        self.atom_total[row] = mass.sum() / (n_t * tot) if tot > 0 else 0.0
+        self.nout[row] = G
+        self.spans[row] = [(int(panel.comp_slot_begin[b[0]]) - self._base,
+                            int(panel.comp_slot_end[b[-1]]) - self._base) for b in groups]
+        return row
+
+    def release(self, row: int) -> None:
+        self.index[row] = None
+        with self._lock:
+            self._free.append(int(row))
 
 
 # ---------------------------------------------------------------- batcher
@@ -261,7 +374,7 @@ class PassBatcher:
             pieces = []
             for r in rs:
                 k = int(r.V.shape[1])
-                r.allocate(self.panel.n_components)
+                r.allocate(int(self.stats.nout[r.trait]))
                 for c0 in range(0, k, self.max_cols):
                     pieces.append((r, c0, min(k, c0 + self.max_cols)))
             group, width = [], 0
@@ -303,20 +416,22 @@ class PassBatcher:
             c = 0
             for r, c0, c1 in group:
                 w = c1 - c0
-                spans = [(int(panel.comp_slot_begin[g]) - base, int(panel.comp_slot_end[g]) - base)
-                         for g in range(G)]
+                spans = st.spans[r.trait]
                 r.take_grams(lambda g, c=c, w=w, spans=spans: T[spans[g][0]:spans[g][1], c:c + w],
                              st.eff[r.trait])
                 c += w
             self.n_calls += 1
             self.columns += width
             return
-        out = torch.empty((G, n, width), dtype=torch.float32, device=dev)
+        cmap = st.comp_map
+        nout = max(int(st.nout[r.trait]) for r, _, _ in group) if cmap is not None else G
+        out = torch.empty((nout, n, width), dtype=torch.float32, device=dev)
         nb = panel.workspace_bytes(N.OP_KV, precision, width, 0, G)
         ws = torch.empty(max(nb, 256), dtype=torch.uint8, device=dev)
         N.call("coh_kv_sets", panel.handle, int(precision), N.ptr(V), int(width), int(V.stride(0)),
                0, int(G), int(st.n_subsets), N.ptr(st.mean), N.ptr(st.inv), int(st.stride),
-               N.ptr(col_set_dev), N.ptr(st.comp_scale), 1, N.ptr(out), int(out.stride(1)),
+               N.ptr(col_set_dev), N.ptr(st.comp_scale), N.ptr(cmap),
+               int(nout) if cmap is not None else 0, 1, N.ptr(out), int(out.stride(1)),
                int(out.stride(0)), 0.0, N.ptr(ws), nb, N.stream_ptr())
         c = 0
         for r, c0, c1 in group:
@@ -348,8 +463,8 @@ class TraitView:
         self.m = int(batcher.streamer.m)
         self.dev = batcher.panel.device
         self.precision = precision
-        self._G = int(batcher.panel.n_components)
-        self._eff = st.eff[trait].astype(np.float64)
+        self._G = int(st.nout[trait])
+        self._eff = np.asarray(st.eff[trait], np.float64)
         self._atoms = st.atoms[trait]
 
     # streamer duck type

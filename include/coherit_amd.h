@@ -164,14 +164,20 @@ int coh_kv(const coh_panel* panel, int32_t precision, const float* dev_V, int64_
  * 1/eff of component g under subset s).  With V's rows outside subset s
  * zero, column c restricted to the subset's rows is what a streamer built
  * with that sample_mask (geno_stream.py:1161-1247) returns from the same
- * operator call (kv_impl.py:1042-1187).  No counterpart in the reference,
- * which runs one fit per mask; int8-limb precisions only (the plain FP32
- * path takes one set through coh_panel_set_standardization).  Workspace:
- * coh_workspace_bytes(COH_OP_KV). */
+ * operator call (kv_impl.py:1042-1187).  With dev_set_comp_map (NULL: one
+ * stacked output per panel component) each set also groups the panel
+ * components of the call into its own nout output components:
+ * dev_set_comp_map[s * (comp_end - comp_begin) + g] is panel component g's
+ * output (-1: left out); a request with its own contiguous SNP partition
+ * built from the panel's blocks (the serving form).  No counterpart in the
+ * reference, which runs one fit per mask; int8-limb precisions only (the
+ * plain FP32 path takes one set through coh_panel_set_standardization).
+ * Workspace: coh_workspace_bytes(COH_OP_KV). */
 int coh_kv_sets(const coh_panel* panel, int32_t precision, const float* dev_V, int64_t k,
                 int64_t ldv, int32_t comp_begin, int32_t comp_end, int32_t nsets,
                 const float* dev_set_mean, const float* dev_set_inv, int64_t set_stride,
-                const int32_t* dev_col_set, const float* dev_set_comp_scale, int32_t stacked,
+                const int32_t* dev_col_set, const float* dev_set_comp_scale,
+                const int32_t* dev_set_comp_map, int32_t nout, int32_t stacked,
                 float* dev_out, int64_t ldo, int64_t out_stride, float beta,
                 void* dev_workspace, uint64_t workspace_bytes, void* stream);
 

@@ -1,0 +1,102 @@
+"""GPU: the serving form.  Requests with their own sample subsets and SNP
+partitions (runs of panel blocks) arrive over time and share panel passes.
+Each served fit equals the same request served alone, and the FP64 oracle
+fit on the request's samples and SNP columns (split-form bar |dh2| <= 1e-3).
+"""
+import numpy as np
+import pytest
+
+from oracle import geno
+from oracle import reml as oreml
+from tests.helpers import int8_block, random_codes
+
+pytestmark = pytest.mark.gpu
+
+BLOCK = 250
+NBLK = 8
+
+
+def _requests(codes, seed):
+    from coherit_amd.serving import FitRequest
+
+    rng = np.random.default_rng(seed)
+    n = codes.shape[0]
+    o = geno.OraclePanel.from_codes(codes, None)
+    Z = o.standardized()
+    C = rng.standard_normal((n, 3))
+    specs = [(0.2, [[0, 1, 2, 3], [4, 5, 6, 7]], 0.00, 0.4),
+             (0.3, [[0], [1, 2], [3, 4, 5, 6, 7]], 0.00, 0.6),
+             (0.1, [[0, 1, 2, 3, 4, 5, 6, 7]], 0.05, 0.3),
+             (0.4, [[2, 3, 4, 5]], 0.10, 0.5)]
+    reqs = []
+    for drop, groups, arrival, h2 in specs:
+        keep = rng.random(n) >= drop
+        cols = np.concatenate([np.arange(b * BLOCK, (b + 1) * BLOCK) for g in groups for b in g])
+        zb = Z[:, cols] @ rng.standard_normal(cols.size)
+        p = 1 + int(rng.integers(1, 3))
+        X = np.concatenate([np.ones((n, 1)), C[:, :p]], axis=1)
+        y = X @ rng.standard_normal(p + 1) * 0.3 + np.sqrt(h2) * zb / zb.std() \
+            + np.sqrt(1 - h2) * rng.standard_normal(n)
+        reqs.append(FitRequest(keep=keep, groups=groups, y=y.astype(np.float32),
+                               X=X.astype(np.float32), arrival_s=arrival))
+    return reqs
+
+
+def _h2(theta, atoms):
+    G = len(atoms)
+    gen = float(np.dot(theta[:G], atoms))
+    return gen / (gen + float(theta[G]))
+
+
+def test_served_fits_equal_alone_and_match_oracle():
+    import dataclasses
+
+    from coherit_amd.geno_stream import GenoBlockStreamer
+    from coherit_amd.reml_model import FitConfig
+    from coherit_amd.serving import FitServer, latency_summary
+    from coherit_amd.sources import ArraySource
+
+    n = 500
+    codes = random_codes(n, BLOCK * NBLK, seed=51)
+    st = GenoBlockStreamer(ArraySource(int8_block(codes), missing_val=-9), call_width=BLOCK,
+                           component_block_sizes=[BLOCK] * NBLK, precision="i8x2")
+    kw = dict(precond_rank=30, slq_samples=8, slq_m=12, minq_iter=12, reml_pcg_tol=1e-5, seed=7)
+    cfg = FitConfig(precision="i8x2", verbose=False, **kw)
+    server = FitServer(st, cfg, capacity=8, max_cols=512)
+    reqs = _requests(codes, seed=52)
+    mix = server.run(reqs, keep_precond=True)
+    print("mix", latency_summary(mix), "batches", server.last_batcher.n_batches,
+          "requests", server.last_batcher.requests, flush=True)
+    for i, req in enumerate(reqs):
+        f = mix[i]
+        assert f.error is None, f.error
+        alone = server.run([dataclasses.replace(req, arrival_s=0.0)])[0]
+        assert alone.error is None, alone.error
+        # the oracle on the request's samples and SNP columns
+        cols = np.concatenate([np.arange(b * BLOCK, (b + 1) * BLOCK)
+                               for g in req.groups for b in g])
+        sizes = [len(g) * BLOCK for g in req.groups]
+        o = geno.OraclePanel.from_codes(codes[req.keep][:, cols], sizes)
+        Zs = o.standardized()
+        off = o.component_offsets
+        Ks = [Zs[:, off[g]:off[g + 1]] @ Zs[:, off[g]:off[g + 1]].T / float(o.atoms.component_eff[g])
+              for g in range(len(sizes))]
+        atoms = o.atoms.component_mean_diag.astype(np.float64)
+        pb = oreml.Problem(Ks, atoms)
+        idx = np.flatnonzero(req.keep)
+        if len(sizes) > 1:
+            th_o = oreml.fit_reml(pb, req.y[idx], req.X[idx], seed=7, slq_samples=8, slq_m=12,
+                                  minq_iter=12, pcg_tol=1e-5, slq_mode="projected_core_residual",
+                                  precond_U=f.precond_U.double().cpu().numpy())
+        else:
+            th_o = oreml.fit_reml(pb, req.y[idx], req.X[idx], seed=7, slq_samples=8, slq_m=12,
+                                  minq_iter=12, pcg_tol=1e-5, maxiter=400)
+        print("request", i, "G", len(sizes), "n_t", f.n_samples, "mix", f.result.var_components,
+              f.result.heritability, "alone", alone.result.var_components,
+              alone.result.heritability, "oracle", th_o, _h2(th_o, atoms), flush=True)
+        np.testing.assert_allclose(f.result.genetic_trace_atoms, atoms, rtol=1e-6)
+        assert abs(f.result.heritability - alone.result.heritability) <= 1e-6
+        np.testing.assert_allclose(f.result.var_components, alone.result.var_components,
+                                   rtol=1e-5, atol=1e-7)
+        assert abs(f.result.heritability - _h2(th_o, atoms)) <= 1e-3
+    st.close()
