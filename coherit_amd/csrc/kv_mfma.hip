@@ -670,8 +670,11 @@ __global__ __launch_bounds__(256) void k_p2_mfma(P2Args a) {
 template <int L>
 __global__ void k_p2_reduce(const int32_t* __restrict__ part, int64_t n, int64_t k,
                             const double* __restrict__ scale, const double* __restrict__ wsum,
-                            ChunkTable tab, int nout, int stacked, StatSets ss,
-                            float* __restrict__ out, int64_t ldo, int64_t out_stride, float beta) {
+                            ChunkTable tab, int g_base, int ncomp_all, int nout, int stacked,
+                            StatSets ss, float* __restrict__ out, int64_t ldo, int64_t out_stride,
+                            float beta) {
+  // tab: the chunks of components g_base .. g_base + tab.ncomp of the call
+  // (ncomp_all components); scales and mean terms are indexed by call component
   const int64_t total = (int64_t)nout * n * k;
   for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
        idx += (int64_t)gridDim.x * blockDim.x) {
@@ -680,7 +683,9 @@ __global__ void k_p2_reduce(const int32_t* __restrict__ part, int64_t n, int64_t
     const int go = static_cast<int>(idx / (k * n));
     // stacked outputs: one per component, or the column's set's grouping
     const int32_t* cmap =
-        (stacked && ss.comp_map) ? ss.comp_map + (int64_t)ss.set_of(col) * tab.ncomp : nullptr;
+        (stacked && ss.comp_map)
+            ? ss.comp_map + (int64_t)ss.set_of(col) * ncomp_all + g_base
+            : nullptr;
     const int g0 = (stacked && !cmap) ? go : 0;
     const int g1 = (stacked && !cmap) ? go + 1 : tab.ncomp;
     double acc = 0.0;
@@ -695,10 +700,12 @@ __global__ void k_p2_reduce(const int32_t* __restrict__ part, int64_t n, int64_t
       double t = 0.0;
 #pragma unroll
       for (int l = 0; l < L; ++l) t += ldexp(static_cast<double>(s[l]), 8 * l);
-      acc += scale[(int64_t)g * k + col] * t - wsum[(int64_t)g * k + col];
+      const int64_t gg = g_base + g;
+      acc += scale[gg * k + col] * t - wsum[gg * k + col];
     }
     const float v = static_cast<float>(acc);
-    float* o = out + (int64_t)go * out_stride + i * ldo + col;
+    const int64_t oi = (stacked && !cmap) ? g_base + go : go;
+    float* o = out + oi * out_stride + i * ldo + col;
     *o = (beta != 0.0f) ? beta * (*o) + v : v;
   }
 }
@@ -745,8 +752,13 @@ static int64_t col_blocks(int64_t k, int NB) { return ceil_div(ceil_div(k, 16), 
 
 constexpr int kColmaxBlocks = 256;
 
+// Bound on the phase-2 int32 chunk partials (bytes); more components are
+// processed in successive groups.
+constexpr int64_t kP2PartialCap = int64_t(2) << 30;
+
 int plan_mfma(const coh_panel* p, int precision, int64_t k, int32_t cb, int32_t ce, KvPlan* pl) {
   pl->limbs = (precision == COH_PREC_I8X3) ? 3 : 2;
+  pl->comp0 = cb;
   pl->slot0 = p->comp_slot_begin[cb];
   pl->slot1 = p->comp_slot_end[ce - 1];
   const int64_t nslots = pl->slot1 - pl->slot0;
@@ -796,7 +808,31 @@ int plan_mfma(const coh_panel* p, int precision, int64_t k, int32_t cb, int32_t 
          round_up((int64_t)kColmaxBlocks * k * 8, 256);
   pl->off_p1 = off;
   int64_t p1 = (nsplit > 1) ? nsplit * nslots * k * 2 * L * 4 : 0;
-  int64_t p2 = (int64_t)pl->tab.nchunks * p->n * k * L * 4;
+  // phase-2 component groups: all components at once while their chunk
+  // partials fit kP2PartialCap, else one component per group (a rule that
+  // does not depend on where the cap falls, so a column's sums over
+  // components always run in the same order for a given panel and size
+  // class; sums over groups accumulate in FP32, in component order)
+  const int64_t per_chunk = p->n * k * L * 4;
+  int64_t cap = kP2PartialCap;
+  if (const char* e = std::getenv("COH_P2_PARTIAL_CAP")) {  // testing / tuning
+    const long long v = std::atoll(e);
+    if (v > 0) cap = v;
+  }
+  const int64_t all = (int64_t)pl->tab.nchunks * per_chunk;
+  int64_t p2 = 0;
+  pl->group_begin[0] = 0;
+  if (all <= cap || pl->tab.ncomp == 1) {
+    pl->ngroups = 1;
+    pl->group_begin[1] = pl->tab.ncomp;
+    p2 = all;
+  } else {
+    pl->ngroups = pl->tab.ncomp;
+    for (int g = 0; g < pl->tab.ncomp; ++g) {
+      pl->group_begin[g + 1] = g + 1;
+      p2 = std::max(p2, (int64_t)(pl->tab.chunk_off[g + 1] - pl->tab.chunk_off[g]) * per_chunk);
+    }
+  }
   off += round_up(std::max(p1, p2), 256);
   pl->total = off;
   return COH_OK;
@@ -952,39 +988,60 @@ static int phase2(const coh_panel* p, const KvPlan& pl, int64_t k, const StatSet
   a.k = k;
   a.Bu = Bu;
   a.scale = scale;
-  a.tab = pl.tab;
   a.part = reinterpret_cast<int32_t*>(ws + pl.off_p1);
   a.ux = ceil_div(p->n, 64 * MB);
-  a.uy = pl.tab.nchunks;
   a.groups = (int)groups;
-  dim3 grid(xcd_grid(a.ux, a.uy, groups));
   char name[64];
   snprintf(name, sizeof(name), "k_p2_mfma_L%d_NB%d_MB%d_T%d_PD%d", L, NB, MB, T, PD);
-  KernelTimer tm(st, name, (double)p->n * (double)ceil_div(pl.m_sel, 4),
-                 2.0 * (double)p->n * (double)pl.m_sel * (double)k,
-                 2.0 * L * 2.0 * (double)nslots * (double)round_up(p->n, 64 * MB) *
-                     (double)(NBt * 16));
-  bool launched = false;
+  const int ncomp_all = pl.tab.ncomp;
+  for (int gi = 0; gi < pl.ngroups; ++gi) {
+    const int ga = pl.group_begin[gi], gb = pl.group_begin[gi + 1];
+    // the group's chunks, numbered from 0
+    ChunkTable sub{};
+    sub.ncomp = gb - ga;
+    sub.chunk_len = pl.tab.chunk_len;
+    for (int g = 0; g < sub.ncomp; ++g) {
+      sub.begin[g] = pl.tab.begin[ga + g];
+      sub.end[g] = pl.tab.end[ga + g];
+      sub.chunk_off[g] = pl.tab.chunk_off[ga + g] - pl.tab.chunk_off[ga];
+    }
+    sub.chunk_off[sub.ncomp] = pl.tab.chunk_off[gb] - pl.tab.chunk_off[ga];
+    sub.nchunks = sub.chunk_off[sub.ncomp];
+    a.tab = sub;
+    a.uy = sub.nchunks;
+    const int64_t gslots = sub.end[sub.ncomp - 1] - sub.begin[0];
+    const int64_t gsnps = selected_snps(p, pl.comp0 + ga, pl.comp0 + gb);
+    dim3 grid(xcd_grid(a.ux, a.uy, groups));
+    KernelTimer tm(st, name, (double)p->n * (double)ceil_div(gsnps, 4),
+                   2.0 * (double)p->n * (double)gsnps * (double)k,
+                   2.0 * L * 2.0 * (double)gslots * (double)round_up(p->n, 64 * MB) *
+                       (double)(NBt * 16));
+    bool launched = false;
 #define COH_P2V(NB_, MB_, T_, PD_)                                                      \
   if constexpr (L * NB_ * T_ <= 16)                                                     \
     if (!launched && NB == NB_ && MB == MB_ && T == T_ && PD == PD_) {                  \
       hipLaunchKernelGGL((k_p2_mfma<L, NB_, MB_, T_, PD_>), grid, dim3(256), 0, st, a); \
       launched = true;                                                                  \
     }
-  COH_P2V(1, 8, 4, 1) COH_P2V(1, 4, 4, 1) COH_P2V(1, 8, 2, 1) COH_P2V(2, 4, 1, 1) COH_P2V(2, 4, 2, 1)
-  COH_P2V(2, 8, 1, 1) COH_P2V(2, 8, 2, 1) COH_P2V(4, 2, 1, 1) COH_P2V(4, 2, 2, 1) COH_P2V(4, 4, 1, 1)
-  COH_P2V(4, 4, 2, 1) COH_P2V(4, 1, 2, 1)
-  COH_P2V(1, 4, 4, 2) COH_P2V(1, 4, 4, 3) COH_P2V(1, 8, 4, 2) COH_P2V(2, 4, 2, 2)
-  COH_P2V(2, 4, 2, 3) COH_P2V(4, 4, 2, 2) COH_P2V(4, 2, 2, 2)
+    COH_P2V(1, 8, 4, 1) COH_P2V(1, 4, 4, 1) COH_P2V(1, 8, 2, 1) COH_P2V(2, 4, 1, 1) COH_P2V(2, 4, 2, 1)
+    COH_P2V(2, 8, 1, 1) COH_P2V(2, 8, 2, 1) COH_P2V(4, 2, 1, 1) COH_P2V(4, 2, 2, 1) COH_P2V(4, 4, 1, 1)
+    COH_P2V(4, 4, 2, 1) COH_P2V(4, 1, 2, 1)
+    COH_P2V(1, 4, 4, 2) COH_P2V(1, 4, 4, 3) COH_P2V(1, 8, 4, 2) COH_P2V(2, 4, 2, 2)
+    COH_P2V(2, 4, 2, 3) COH_P2V(4, 4, 2, 2) COH_P2V(4, 2, 2, 2)
 #undef COH_P2V
-  if (!launched) return fail(COH_ERR_VALUE, "no compiled phase-2 kernel for this MFMA shape");
-  tm.stop();
-  COH_CHECK_LAUNCH();
-  const int nout = stacked ? (ss.comp_map ? ss.nout : pl.tab.ncomp) : 1;
-  hipLaunchKernelGGL(k_p2_reduce<L>, dim3(grid1d((int64_t)nout * p->n * k)), dim3(256), 0, st,
-                     a.part, p->n, k, scale, sv.wsum, pl.tab, nout, stacked, ss, out, ldo,
-                     out_stride, beta);
-  COH_CHECK_LAUNCH();
+    if (!launched) return fail(COH_ERR_VALUE, "no compiled phase-2 kernel for this MFMA shape");
+    tm.stop();
+    COH_CHECK_LAUNCH();
+    // outputs: stacked per component are the group's own (caller's beta);
+    // a sum over components or a set's grouping accumulates over groups
+    const bool own = stacked && !ss.comp_map;
+    const int nout = stacked ? (ss.comp_map ? ss.nout : sub.ncomp) : 1;
+    const float b = (own || gi == 0) ? beta : 1.0f;
+    hipLaunchKernelGGL(k_p2_reduce<L>, dim3(grid1d((int64_t)nout * p->n * k)), dim3(256), 0, st,
+                       a.part, p->n, k, scale, sv.wsum, sub, ga, ncomp_all, nout, stacked, ss,
+                       out, ldo, out_stride, b);
+    COH_CHECK_LAUNCH();
+  }
   return COH_OK;
 }
 

@@ -89,6 +89,12 @@ struct KvPlan {
   int64_t n_pad;            // samples padded to the MFMA K tile
   int64_t m_sel;            // real SNPs of the requested components
   int shape[8];             // MFMA launch shape: NB, MB, T, PD of phase 1 then phase 2
+  // phase 2 runs over groups of whole components whose int32 chunk partials
+  // fit the partial buffer: group i = components [group_begin[i], group_begin[i+1])
+  // of the plan (0-based within [cb, ce))
+  int ngroups;
+  int group_begin[kMaxComp + 1];
+  int32_t comp0;            // first component of the call
 };
 
 inline int64_t selected_snps(const coh_panel* p, int32_t cb, int32_t ce) {

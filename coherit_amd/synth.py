@@ -160,6 +160,58 @@ def trait_phenotypes(streamer, wl: Workload, n_traits: int, *, seed: int | None 
     return Y.astype(np.float32), X.astype(np.float32), keep, h2
 
 
+def serving_requests(streamer, n_requests: int, rate_per_hour: float, *, seed: int = 20261207,
+                     n_blocks: int | None = None, median_samples: float = 50_000.0,
+                     sigma: float = 1.0, min_samples: int = 10_000, max_components: int = 8,
+                     covariates=(5, 20), h2_range=(0.1, 0.7)):
+    """Poisson-arriving REML fit requests on a panel of contiguous blocks.
+
+    Each request: a sample subset of size LogNormal(ln median, sigma) clipped
+    to [min_samples, n]; 1..max_components components that are a random
+    partition of the blocks into runs; 5..20 N(0,1) covariates plus an
+    intercept; h2 ~ U(h2_range) carried by 1% causal SNPs of its components
+    (N(0,1) effects, one Z.B pass for all requests).  Returns a list of
+    ``serving.FitRequest`` sorted by arrival.
+    """
+    from .serving import FitRequest
+
+    rng = np.random.default_rng(seed)
+    n, R = streamer.n, int(n_requests)
+    B = int(streamer.n_components if n_blocks is None else n_blocks)
+    off = streamer._component_snp_offsets
+    arrivals = np.cumsum(rng.exponential(3600.0 / float(rate_per_hour), size=R))
+    arrivals -= arrivals[0]
+    C = rng.standard_normal((n, int(covariates[1])))
+    Bmat = np.zeros((streamer.m, R), np.float32)
+    specs = []
+    for r in range(R):
+        size = int(np.clip(np.exp(rng.normal(np.log(median_samples), sigma)), min_samples, n))
+        keep = np.zeros(n, bool)
+        keep[rng.choice(n, size=size, replace=False)] = True
+        G = int(rng.integers(1, min(max_components, B) + 1))
+        cuts = np.sort(rng.choice(np.arange(1, B), size=G - 1, replace=False)) if G > 1 else []
+        edges = np.concatenate([[0], cuts, [B]]).astype(int)
+        groups = [list(range(edges[g], edges[g + 1])) for g in range(G)]
+        snps = np.arange(int(off[0]), int(off[B]))
+        k = max(1, int(round(0.01 * snps.size)))
+        causal = rng.choice(snps, size=k, replace=False)
+        Bmat[causal, r] = rng.standard_normal(k)
+        p = int(rng.integers(int(covariates[0]), int(covariates[1]) + 1))
+        h2 = float(rng.uniform(*h2_range))
+        specs.append((keep, groups, p, h2))
+    gen = streamer.panel.zb(torch.as_tensor(Bmat), 0, streamer.n_components,
+                            precision=0).double().cpu().numpy()
+    out = []
+    for r, (keep, groups, p, h2) in enumerate(specs):
+        g = gen[:, r]
+        g = (g - g.mean()) / max(float(g.std()), 1e-12)
+        X = np.concatenate([np.ones((n, 1)), C[:, :p]], axis=1)
+        y = X @ rng.standard_normal(p + 1) + np.sqrt(h2) * g + np.sqrt(1 - h2) * rng.standard_normal(n)
+        out.append(FitRequest(keep=keep, groups=groups, y=y.astype(np.float32),
+                              X=X.astype(np.float32), arrival_s=float(arrivals[r])))
+    return out
+
+
 def initial_components(y, X, atoms, h2_init: float = 0.5) -> np.ndarray:
     """Start point (theta_g..., theta_e) scaled to the OLS residual variance.
 

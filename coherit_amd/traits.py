@@ -318,6 +318,9 @@ class PassBatcher:
         self.busy_s = 0.0
 
     def submit(self, req: _Request) -> torch.Tensor:
+        """Called from a fit's thread.  All fits enqueue on the same device
+        stream, so the shared contraction is ordered after every request's
+        inputs and before every later use of its results."""
         with self._cv:
             self._pending.append(req)
             if len(self._pending) >= self._live:
