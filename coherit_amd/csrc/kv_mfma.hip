@@ -531,8 +531,17 @@ struct P2Args {
   int groups;          // column groups
 };
 
+// Waves per SIMD the phase-2 kernel is compiled for: 2 for the four-block,
+// single-step stage shapes (measured on MI355X: the compiler's one-wave
+// schedule of those is far slower), else the compiler's choice.
+template <int NB, int T>
+constexpr int p2_waves() {
+  return (NB == 4 && T == 1) ? 2 : 1;
+}
+
 template <int L, int NB, int MB, int T, int PD>
-__global__ __launch_bounds__(256) void k_p2_mfma(P2Args a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(p2_waves<NB, T>())))
+void k_p2_mfma(P2Args a) {
   // Double-buffered LDS stages of T K-steps (u and mean*u halves).
   constexpr int STG = L * NB * T * 2 * 64;
   constexpr int PER = (STG + 255) / 256;
@@ -616,28 +625,53 @@ __global__ __launch_bounds__(256) void k_p2_mfma(P2Args a) {
 #pragma unroll
       for (int tt = 0; tt < T; ++tt) {
         const int t = st * T + tt;
-        v4i bu[L][NB], bw[L][NB];
+        if constexpr (p2_waves<NB, T>() == 2) {
+          // two waves per SIMD: decode the row tiles once, then stream the B
+          // fragments per (limb, column block), few B registers live at a time
+          v4i ag[MB], am[MB];
 #pragma unroll
-        for (int l = 0; l < L; ++l)
-#pragma unroll
-          for (int nb = 0; nb < NB; ++nb) {
-            const int base = (((l * NB + nb) * T + tt) * 2) * 64 + lane;
-            bu[l][nb] = sB[buf][base];
-            bw[l][nb] = sB[buf][base + 64];
-          }
-#pragma unroll
-        for (int mb = 0; mb < MB; ++mb) {
-          v4i ag, am;
-          decode16m(static_cast<uint32_t>(pipe[0][mb][t]), ag, am);
+          for (int mb = 0; mb < MB; ++mb)
+            decode16m(static_cast<uint32_t>(pipe[0][mb][t]), ag[mb], am[mb]);
 #pragma unroll
           for (int l = 0; l < L; ++l)
 #pragma unroll
             for (int nb = 0; nb < NB; ++nb) {
-              acc[l][mb][nb] =
-                  __builtin_amdgcn_mfma_i32_16x16x64_i8(ag, bu[l][nb], acc[l][mb][nb], 0, 0, 0);
-              acc[l][mb][nb] =
-                  __builtin_amdgcn_mfma_i32_16x16x64_i8(am, bw[l][nb], acc[l][mb][nb], 0, 0, 0);
+              const int base = (((l * NB + nb) * T + tt) * 2) * 64 + lane;
+              const v4i bu = sB[buf][base];
+              const v4i bw = sB[buf][base + 64];
+#pragma unroll
+              for (int mb = 0; mb < MB; ++mb) {
+                acc[l][mb][nb] =
+                    __builtin_amdgcn_mfma_i32_16x16x64_i8(ag[mb], bu, acc[l][mb][nb], 0, 0, 0);
+                acc[l][mb][nb] =
+                    __builtin_amdgcn_mfma_i32_16x16x64_i8(am[mb], bw, acc[l][mb][nb], 0, 0, 0);
+              }
             }
+        } else {
+          // one wave per SIMD: all B fragments of the K-step, then per row tile
+          v4i bu[L][NB], bw[L][NB];
+#pragma unroll
+          for (int l = 0; l < L; ++l)
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb) {
+              const int base = (((l * NB + nb) * T + tt) * 2) * 64 + lane;
+              bu[l][nb] = sB[buf][base];
+              bw[l][nb] = sB[buf][base + 64];
+            }
+#pragma unroll
+          for (int mb = 0; mb < MB; ++mb) {
+            v4i ag, am;
+            decode16m(static_cast<uint32_t>(pipe[0][mb][t]), ag, am);
+#pragma unroll
+            for (int l = 0; l < L; ++l)
+#pragma unroll
+              for (int nb = 0; nb < NB; ++nb) {
+                acc[l][mb][nb] =
+                    __builtin_amdgcn_mfma_i32_16x16x64_i8(ag, bu[l][nb], acc[l][mb][nb], 0, 0, 0);
+                acc[l][mb][nb] =
+                    __builtin_amdgcn_mfma_i32_16x16x64_i8(am, bw[l][nb], acc[l][mb][nb], 0, 0, 0);
+              }
+          }
         }
       }
       sstore(buf ^ 1);
@@ -730,7 +764,7 @@ struct MfmaShape {
 static MfmaShape default_shape(int64_t k, int L) {
   if (k <= 16) return {1, 4, 4, 2, 1, 4, 4, 1};
   if (k <= 32) return {2, 2, 4, 1, 2, 4, 2, 1};
-  if (L == 2) return {2, 2, 4, 1, 4, 4, 2, 1};
+  if (L == 2) return {2, 2, 4, 1, 4, 4, 1, 1};
   return {2, 2, 4, 1, 2, 4, 2, 1};
 }
 
