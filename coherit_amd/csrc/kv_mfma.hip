@@ -324,8 +324,17 @@ struct P1Args {
   int groups;          // column groups
 };
 
+// Waves per SIMD the phase-1 kernel is compiled for: 2 for the four-block
+// shapes with two or more row tiles (their one-wave build needs 350+
+// registers), else the compiler's choice.
+template <int NB, int MB>
+constexpr int p1_waves() {
+  return (NB == 4 && MB >= 2) ? 2 : 1;
+}
+
 template <int L, int NB, int MB, int T, int PD>
-__global__ __launch_bounds__(256) void k_p1_mfma(P1Args a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(p1_waves<NB, MB>())))
+void k_p1_mfma(P1Args a) {
   // Double-buffered LDS stages of T K-steps of the quantised B block, shared
   // by the 4 waves; the next stage's global loads overlap this stage's MFMAs.
   // The genotype rows (A) stream from HBM with no reuse: each wave keeps PD
@@ -416,24 +425,48 @@ __global__ __launch_bounds__(256) void k_p1_mfma(P1Args a) {
 #pragma unroll
       for (int tt = 0; tt < T; ++tt) {
         const int t = st * T + tt;
-        v4i b[L][NB];
+        if constexpr (p1_waves<NB, MB>() == 2) {
+          // two waves per SIMD: decode the row tiles once, then stream the B
+          // fragments per (limb, column block)
+          v4i ag[MB], ao[MB];
 #pragma unroll
-        for (int l = 0; l < L; ++l)
-#pragma unroll
-          for (int nb = 0; nb < NB; ++nb) b[l][nb] = sB[buf][((l * NB + nb) * T + tt) * 64 + lane];
-#pragma unroll
-        for (int mb = 0; mb < MB; ++mb) {
-          v4i ag, ao;
-          decode16(static_cast<uint32_t>(pipe[0][mb][t]), ag, ao);
+          for (int mb = 0; mb < MB; ++mb)
+            decode16(static_cast<uint32_t>(pipe[0][mb][t]), ag[mb], ao[mb]);
 #pragma unroll
           for (int l = 0; l < L; ++l)
 #pragma unroll
             for (int nb = 0; nb < NB; ++nb) {
-              accg[l][mb][nb] =
-                  __builtin_amdgcn_mfma_i32_16x16x64_i8(ag, b[l][nb], accg[l][mb][nb], 0, 0, 0);
-              acco[l][mb][nb] =
-                  __builtin_amdgcn_mfma_i32_16x16x64_i8(ao, b[l][nb], acco[l][mb][nb], 0, 0, 0);
+              const v4i b = sB[buf][((l * NB + nb) * T + tt) * 64 + lane];
+#pragma unroll
+              for (int mb = 0; mb < MB; ++mb) {
+                accg[l][mb][nb] =
+                    __builtin_amdgcn_mfma_i32_16x16x64_i8(ag[mb], b, accg[l][mb][nb], 0, 0, 0);
+                acco[l][mb][nb] =
+                    __builtin_amdgcn_mfma_i32_16x16x64_i8(ao[mb], b, acco[l][mb][nb], 0, 0, 0);
+              }
             }
+        } else {
+          // all B fragments of the K-step, then per row tile
+          v4i b[L][NB];
+#pragma unroll
+          for (int l = 0; l < L; ++l)
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb)
+              b[l][nb] = sB[buf][((l * NB + nb) * T + tt) * 64 + lane];
+#pragma unroll
+          for (int mb = 0; mb < MB; ++mb) {
+            v4i ag, ao;
+            decode16(static_cast<uint32_t>(pipe[0][mb][t]), ag, ao);
+#pragma unroll
+            for (int l = 0; l < L; ++l)
+#pragma unroll
+              for (int nb = 0; nb < NB; ++nb) {
+                accg[l][mb][nb] =
+                    __builtin_amdgcn_mfma_i32_16x16x64_i8(ag, b[l][nb], accg[l][mb][nb], 0, 0, 0);
+                acco[l][mb][nb] =
+                    __builtin_amdgcn_mfma_i32_16x16x64_i8(ao, b[l][nb], acco[l][mb][nb], 0, 0, 0);
+              }
+          }
         }
       }
       sstore(buf ^ 1);
@@ -764,7 +797,10 @@ struct MfmaShape {
 static MfmaShape default_shape(int64_t k, int L) {
   if (k <= 16) return {1, 4, 4, 2, 1, 4, 4, 1};
   if (k <= 32) return {2, 2, 4, 1, 2, 4, 2, 1};
-  if (L == 2) return {2, 2, 4, 1, 4, 4, 1, 1};
+  // phase 1: two-block groups up to 128 columns (less column padding at the
+  // SLQ widths 64..73), four-block two-wave groups above
+  if (L == 2 && k <= 128) return {2, 2, 4, 1, 4, 4, 1, 1};
+  if (L == 2) return {4, 2, 4, 1, 4, 4, 1, 1};
   return {2, 2, 4, 1, 2, 4, 2, 1};
 }
 
