@@ -82,8 +82,11 @@ class FitServer:
 
     def __init__(self, streamer, cfg: FitConfig | None = None, *, capacity: int = 64,
                  max_cols: int = 1024, hbm_budget_bytes: float | None = None,
-                 headroom: float = 0.85):
+                 headroom: float = 0.85, flush_after_s: float | None = 0.005):
         self.streamer = streamer
+        # a shared pass starts when every admitted fit waits, or when the
+        # oldest waiting request has waited this long
+        self.flush_after_s = flush_after_s
         self.cfg = cfg or FitConfig(precision="i8x2")
         if self.cfg.precision not in ("i8x2", "i8x3"):
             raise ValueError("the server runs the int8-limb forms ('i8x2' or 'i8x3').")
@@ -106,7 +109,7 @@ class FitServer:
         whitening basis when ``keep_precond``)."""
         cfg = self.cfg
         batcher = PassBatcher(self.streamer, self.pool, 0, max_cols=self.max_cols,
-                              progress=progress)
+                              progress=progress, flush_after_s=self.flush_after_s)
         order = sorted(range(len(requests)), key=lambda i: requests[i].arrival_s)
         rank = {i: q for q, i in enumerate(order)}
         gate = threading.Condition()

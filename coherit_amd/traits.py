@@ -180,6 +180,16 @@ class StatSetPool:
             if not self._free:
                 raise RuntimeError("no free statistic set (capacity reached).")
             row = self._free.pop()
+        try:
+            self._fill(row, keep, groups, cmap)
+        except BaseException:
+            self.release(row)
+            raise
+        return row
+
+    def _fill(self, row, keep, groups, cmap) -> None:
+        panel = self.panel
+        Gp = int(panel.n_components)
         dev = panel.device
         # the subset stats run on the calling thread's stream, in order with
         # the request's later operator calls
@@ -215,7 +225,6 @@ class StatSetPool:
         self.nout[row] = G
         self.spans[row] = [(int(panel.comp_slot_begin[b[0]]) - self._base,
                             int(panel.comp_slot_end[b[-1]]) - self._base) for b in groups]
-        return row
 
     def release(self, row: int) -> None:
         self.index[row] = None
@@ -300,8 +309,11 @@ class PassBatcher:
     """
 
     def __init__(self, streamer, stats: SubsetStats, n_fits: int, *, max_cols: int = 2048,
-                 progress=None, progress_every_s: float = 20.0):
+                 progress=None, progress_every_s: float = 20.0,
+                 flush_after_s: float | None = None):
         self.streamer = streamer
+        self.flush_after_s = flush_after_s
+        self._first_pending = None
         self.progress = progress
         self.progress_every_s = float(progress_every_s)
         self._t_start = self._t_report = time.perf_counter()
@@ -322,12 +334,23 @@ class PassBatcher:
         stream, so the shared contraction is ordered after every request's
         inputs and before every later use of its results."""
         with self._cv:
+            if not self._pending:
+                self._first_pending = time.perf_counter()
             self._pending.append(req)
             if len(self._pending) >= self._live:
                 self._run_locked()
             else:
                 while not req.done:
-                    self._cv.wait()
+                    if self.flush_after_s is None:
+                        self._cv.wait()
+                        continue
+                    # a deadline on the oldest pending request: fits busy
+                    # with host work do not hold back the ones that wait
+                    left = self._first_pending + self.flush_after_s - time.perf_counter()
+                    if left <= 0 and self._pending:
+                        self._run_locked()
+                    else:
+                        self._cv.wait(timeout=max(left, 1e-4))
         if req.error is not None:
             raise req.error
         return req.result

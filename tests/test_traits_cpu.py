@@ -157,3 +157,35 @@ def test_gram_requests_divide_each_component_by_its_eff():
     for g in range(G):
         want = T[g].T @ T[g] / eff[g] if eff[g] > 0 else torch.zeros((k, k), dtype=torch.float64)
         torch.testing.assert_close(req.result[g], want, rtol=1e-12, atol=1e-12)
+
+
+def test_flush_deadline_lets_waiting_fits_run_without_the_busy_one():
+    """With a deadline, a fit busy with long host work does not hold back the
+    other: its requests run in smaller batches, and every result is right."""
+    log = []
+    b = _batcher(2, log)
+    b.flush_after_s = 0.005
+    out = []
+
+    def fit(t, calls, pause):
+        try:
+            for c in range(calls):
+                time.sleep(pause)
+                V = torch.full((4, 1), float(10 * t + c))
+                out.append((t, c, b.submit(_Request(t, V, "i8x2"))))
+        finally:
+            b.leave()
+
+    t0 = time.perf_counter()
+    th = [threading.Thread(target=fit, args=(0, 20, 0.0)),
+          threading.Thread(target=fit, args=(1, 2, 0.2))]
+    for x in th:
+        x.start()
+    th[0].join(timeout=30)
+    fast_done = time.perf_counter() - t0
+    th[1].join(timeout=30)
+    assert fast_done < 0.3  # the fast fit did not wait for the slow one's pauses
+    assert any(batch == [0] for batch in log)
+    assert len(out) == 22
+    for t, c, res in out:
+        assert torch.equal(res[0], torch.full((4, 1), 2.0 * (10 * t + c)))
