@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--no-plain", action="store_true",
                     help="skip the untimed plain-FP32 comparison fit")
     ap.add_argument("--cpu-slice-snps", type=int, default=8192)
+    ap.add_argument("--cpu-reps", type=int, default=3,
+                    help="timed repetitions of each CPU K.V on the slice (about 10 s in all)")
     ap.add_argument("--kv-only", type=int, default=0,
                     help="skip the fit; time this many K.V launches at PCG/SLQ widths (profiling)")
     ap.add_argument("--calibrate-copy", type=float, default=0.0,
@@ -185,14 +187,16 @@ def cpu_baseline(args, st, call_log):
     mean, inv = ogeno.f32_stats(cnt, s1, s2)
     per_snp = {}
     t_total = 0.0
+    reps = max(1, int(args.cpu_reps))
     for k in (1, 64):
         V = rng.standard_normal((n, k)).astype(np.float32)
         t0 = time.perf_counter()
-        okv.kv_packed_calls(calls, widths, mean, inv, V, float(np.count_nonzero(inv)))
+        for _ in range(reps):
+            okv.kv_packed_calls(calls, widths, mean, inv, V, float(np.count_nonzero(inv)))
         dt = time.perf_counter() - t0
         t_total += dt
-        per_snp[k] = dt / w
-        log(args, f"cpu K.V slice n={n} snps={w} k={k}: {dt:.2f}s ({cores} cores)")
+        per_snp[k] = dt / reps / w
+        log(args, f"cpu K.V slice n={n} snps={w} k={k}: {dt / reps:.2f}s x {reps} ({cores} cores)")
     # linear model per SNP and per K.V: t(k) = a + b k (two contraction phases)
     b = (per_snp[64] - per_snp[1]) / 63.0
     a = per_snp[1] - b
@@ -556,7 +560,8 @@ def main():
         cpu = {"value": round(3600.0 / per_fit, 6), "unit": "fits/hour", "cores": cores,
                "kind": "port",
                "sample": (f"oracle FP32 packed K.V (decode + 2 BLAS GEMMs per 4096-SNP call) on "
-                          f"n={st.n} x {w} SNPs at k=1 and k=64 ({spent:.1f}s), extrapolated "
+                          f"n={st.n} x {w} SNPs at k=1 and k=64, {args.cpu_reps} repetitions "
+                          f"each ({spent:.1f}s), extrapolated "
                           f"linearly in SNPs and width to the fit's {len(call_log)} genotype "
                           f"passes; extrapolated, genotype passes only")}
 
