@@ -1,0 +1,50 @@
+"""CPU: host-side contracts that need no GPU: the bench's single-GPU rule
+and the serving latency summary."""
+import json
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.mark.parametrize("gpus", [2, 8])
+def test_bench_refuses_more_than_one_gpu_with_a_message(gpus):
+    out = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", str(gpus)],
+                         capture_output=True, text=True, timeout=120, cwd=ROOT)
+    assert out.returncode == 2
+    line = json.loads(out.stdout.strip().splitlines()[-1])
+    assert line["n_gpus"] == gpus
+    assert "one GPU" in line["error"]
+
+
+def test_latency_summary_percentiles_and_rate():
+    from coherit_amd.serving import ServedFit, latency_summary
+
+    fits = []
+    for i in range(10):
+        # arrivals every 10 s; each request takes 2 + i seconds, waits i/10 s
+        fits.append(ServedFit(request=i, result=object(), error=None, arrival_s=10.0 * i,
+                              admitted_s=10.0 * i + 0.1 * i, done_s=10.0 * i + 2.0 + i,
+                              n_samples=1000, n_components=1))
+    fits.append(ServedFit(request=10, result=None, error=RuntimeError("x"), arrival_s=5.0,
+                          admitted_s=5.0, done_s=6.0, n_samples=1000, n_components=1))
+    s = latency_summary(fits)
+    lat = np.array([2.0 + i for i in range(10)])
+    assert s["completed"] == 10 and s["failed"] == 1
+    assert s["p50_latency_s"] == pytest.approx(np.percentile(lat, 50))
+    assert s["p99_latency_s"] == pytest.approx(np.percentile(lat, 99))
+    assert s["mean_queue_s"] == pytest.approx(np.mean([0.1 * i for i in range(10)]))
+    # completed fits over the span from the first arrival to the last completion
+    assert s["fits_per_hour"] == pytest.approx(3600.0 * 10 / (90.0 + 11.0))
+
+
+def test_latency_summary_with_no_completed_fit():
+    from coherit_amd.serving import ServedFit, latency_summary
+
+    bad = ServedFit(request=0, result=None, error=RuntimeError("x"), arrival_s=0.0,
+                    admitted_s=0.0, done_s=1.0, n_samples=10, n_components=1)
+    assert latency_summary([bad]) == {"completed": 0}
