@@ -31,8 +31,8 @@ PEAK_FP32_TFLOPS = 157.3       # FP32
 def parse():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--precision", default="i8x2", choices=["fp32", "i8x3", "i8x2"])
     ap.add_argument("--scale", type=float, default=1.0, help="shrink n and m (testing only)")
     ap.add_argument("--precond-rank", type=int, default=500)
@@ -516,9 +516,13 @@ def main():
     N.timing_enable(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    results = [one_fit() for _ in range(args.steps)]
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
+    results, marks = [], []
+    for _ in range(args.steps):
+        results.append(one_fit())
+        torch.cuda.synchronize()
+        marks.append(time.perf_counter())
+    elapsed = marks[-1] - t0
+    log(args, "fit times: " + ", ".join(f"{b - a:.2f}s" for a, b in zip([t0] + marks[:-1], marks)))
     N.timing_enable(False)
     timing = N.timing_read()
     call_log, P.CALL_LOG = P.CALL_LOG, None
@@ -526,7 +530,8 @@ def main():
     ms_per_step = elapsed / args.steps * 1e3
     fits_per_hour = 3600.0 / (elapsed / args.steps)
     name, roof = roofline(timing)
-    kv_calls = sum(1 for c in call_log if c[0] == "kv")
+    # per fit (the call log spans all timed fits)
+    kv_calls = sum(1 for c in call_log if c[0] == "kv") // max(1, args.steps)
     log(args, f"fit: {ms_per_step / 1e3:.1f}s  h2={res.heritability:.4f} (true {wl.h2.sum():.2f}) "
         f"theta={np.round(res.var_components, 5).tolist()} iters={len(res.history)} kv_calls={kv_calls} "
         f"counters={res.counters}")
@@ -562,7 +567,7 @@ def main():
                "sample": (f"oracle FP32 packed K.V (decode + 2 BLAS GEMMs per 4096-SNP call) on "
                           f"n={st.n} x {w} SNPs at k=1 and k=64, {args.cpu_reps} repetitions "
                           f"each ({spent:.1f}s), extrapolated "
-                          f"linearly in SNPs and width to the fit's {len(call_log)} genotype "
+                          f"linearly in SNPs and width to the fit's {len(call_log) // max(1, args.steps)} genotype "
                           f"passes; extrapolated, genotype passes only")}
 
     print(json.dumps({
