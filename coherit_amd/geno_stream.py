@@ -18,6 +18,7 @@ import torch
 from . import _native as N
 from .block_backend import DensePackedBlockDescriptor
 from .panel import ResidentPanel, _dev
+from .precond import tall_t_matmul
 from .sources import BedGenoSource
 from .synth import SyntheticSource
 
@@ -507,7 +508,7 @@ class GenoBlockStreamer:
         if U.ndim != 2:
             raise ValueError("build_projected_core_atom_streamed expects U with shape (n, k).")
         T = self._slot_xtv(U)
-        core = T.T @ T
+        core = tall_t_matmul(T, T)
         return _finalize_core(core, self._eff_m_host, subtract_identity)
 
     def build_projected_core_atoms(self, U, *, subtract_identity: bool = True):
@@ -525,7 +526,7 @@ class GenoBlockStreamer:
             a = int(self._panel.comp_slot_begin[g]) - base
             b = int(self._panel.comp_slot_end[g]) - base
             Tg = T[a:b]
-            cores.append(_finalize_core(Tg.T @ Tg, float(self._component_eff_m_host[g]),
+            cores.append(_finalize_core(tall_t_matmul(Tg, Tg), float(self._component_eff_m_host[g]),
                                         subtract_identity))
         return torch.stack(cores)
 

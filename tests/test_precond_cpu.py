@@ -129,3 +129,21 @@ def test_basis_and_cores_from_grams_equal_basis_then_cores():
     np.testing.assert_allclose((U2 * sign).numpy(), U1.numpy(), atol=1e-5)
     S = sign[None, :, None] * sign[None, None, :]
     np.testing.assert_allclose((cores2.double() * S).numpy(), cores1.numpy(), atol=1e-5)
+
+
+@pytest.mark.parametrize("n,r,k", [(100, 7, 3), (12_800, 40, 9), (12_837, 33, 5), (50_000, 1, 1)])
+def test_split_reduction_product_equals_plain_transposed_product(n, r, k):
+    """tall_t_matmul splits the n-reduction into row chunks (and a tail when
+    n is not a multiple); it is the same product as A^T B, and for FP64
+    input falls back to it exactly."""
+    from coherit_amd.precond import tall_t_matmul
+
+    rng = np.random.default_rng(n + r + k)
+    A = torch.as_tensor(rng.standard_normal((n, r)).astype(np.float32))
+    B = torch.as_tensor(rng.standard_normal((n, k)).astype(np.float32))
+    want = A.double().T @ B.double()
+    got = tall_t_matmul(A, B)
+    assert got.shape == (r, k) and got.dtype == torch.float32
+    torch.testing.assert_close(got.double(), want, rtol=0, atol=2e-5 * float(np.sqrt(n)))
+    A64, B64 = A.double(), B.double()
+    assert torch.equal(tall_t_matmul(A64, B64), A64.T @ B64)
