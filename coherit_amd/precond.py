@@ -14,22 +14,24 @@ import torch
 
 
 def tall_t_matmul(A: torch.Tensor, B: torch.Tensor, chunks: int = 50) -> torch.Tensor:
-    """``A^T B`` for tall FP32 ``A`` (n, r) and ``B`` (n, k).
+    """``A^T B`` for tall ``A`` (n, r) and ``B`` (n, k) of one dtype (FP32 or FP64).
 
     The n-reduction is split into row chunks: one batched product of the
     chunks, then a fixed-order sum of the (chunks, r, k) partials.  For the
     projected-core basis (n = 1e5, r = 500, k = 64) the plain product's
     library kernel takes 290 us against 87 us split 50 ways on MI355X, and
     the shorter FP32 partial sums are also more accurate (7.7e-7 against
-    4.5e-6 relative to an FP64 product).  Other dtypes and short A take the
-    plain product; the partials are capped at 2**24 elements.
+    4.5e-6 relative to an FP64 product).  For the FP64 fixed-effect products
+    (r, k <= 12, n = 1e5) the plain product runs as one workgroup walking
+    the whole reduction (2.7 ms).  Other dtypes and short A take the plain
+    product; the partials are capped at 2**24 elements.
     """
     n = A.shape[0]
     r = A.shape[1] if A.ndim == 2 else 1
     k = B.shape[1] if B.ndim == 2 else 1
     c = min(int(chunks), (1 << 24) // max(1, r * k))
-    if A.dtype != torch.float32 or B.dtype != torch.float32 or A.ndim != 2 or B.ndim != 2 \
-            or c < 2 or n < 64 * c:
+    if A.dtype not in (torch.float32, torch.float64) or B.dtype != A.dtype or A.ndim != 2 \
+            or B.ndim != 2 or c < 2 or n < 64 * c:
         return A.T @ B
     A = A.contiguous()
     B = B.contiguous()

@@ -34,6 +34,7 @@ from .precond import (
     make_precond,
     make_projected_core_precond_from_runtime,
     projected_core_apply_invsqrt,
+    tall_t_matmul,
     projected_core_logdet,
     scalar_diag_from_precond_conf,
 )
@@ -458,13 +459,13 @@ def _eval_once(ctx, pvec, warm_all, warm_ai=None, *, key_slq, minq_tol, maxiter,
     if ctx.xmat is not None and ctx.xmat.shape[1] > 0:
         xc = ctx.xmat.shape[1]
         HinvX = HinvXyZ[:, :xc]
-        XtHinvX = (ctx.xmat.double().T @ HinvX.double()).cpu().numpy()
+        XtHinvX = tall_t_matmul(ctx.xmat.double(), HinvX.double()).cpu().numpy()
         chol = _stable_cho_factor_spd(XtHinvX)
         HinvX64 = HinvX.double()
         X64 = ctx.xmat.double()
 
         def proj(mat):
-            rhs = (X64.T @ mat.double()).cpu().numpy()
+            rhs = tall_t_matmul(X64, mat.double()).cpu().numpy()
             mid = torch.as_tensor(sla.cho_solve(chol, rhs, check_finite=False), device=mat.device)
             return (mat.double() - HinvX64 @ mid).to(torch.float32)
     else:
@@ -503,13 +504,13 @@ def _eval_once(ctx, pvec, warm_all, warm_ai=None, *, key_slq, minq_tol, maxiter,
     if chol is not None:
         if ctx.affine_slq_cache is not None:
             KHX = _apply_genetic_stack(ctx, HinvX)[0]
-            rhs_list = [(HinvX.double().T @ KHX.double()), (HinvX.double().T @ HinvX.double())]
+            rhs_list = [tall_t_matmul(HinvX64, KHX.double()), tall_t_matmul(HinvX64, HinvX64)]
         else:
             stack = _apply_genetic_stack(ctx, HinvX)
-            rhs_list = [HinvX.double().T @ stack[g].double() for g in range(ctx.G)]
+            rhs_list = [tall_t_matmul(HinvX64, stack[g].double()) for g in range(ctx.G)]
             for e in range(ctx.E):
                 prod = HinvX if ctx.residual_diag_stack is None else ctx.residual_diag_stack[e, :, None] * HinvX
-                rhs_list.append(HinvX.double().T @ prod.double())
+                rhs_list.append(tall_t_matmul(HinvX64, prod.double()))
         correction = np.array([np.trace(sla.cho_solve(chol, r.cpu().numpy(), check_finite=False))
                                for r in rhs_list])
         trace_pg = trace_pg - correction
@@ -536,7 +537,7 @@ def _eval_once(ctx, pvec, warm_all, warm_ai=None, *, key_slq, minq_tol, maxiter,
     stats.ai_pcg_iters = int(k_ai)
     stats.ai_elapsed_sec = time.perf_counter() - t0
     PGPy = proj(HinvGPy)
-    AI_sel = 0.5 * (GPy_cols.double().T @ PGPy.double()).cpu().numpy() / scale
+    AI_sel = 0.5 * tall_t_matmul(GPy_cols.double(), PGPy.double()).cpu().numpy() / scale
     AI = 0.5 * (AI_sel + AI_sel.T)
     FI = AverageInfoMatrix(mat=AI, stats=stats)
     grad = 0.5 * (q_sel - trace_pg) / scale

@@ -134,8 +134,8 @@ def test_basis_and_cores_from_grams_equal_basis_then_cores():
 @pytest.mark.parametrize("n,r,k", [(100, 7, 3), (12_800, 40, 9), (12_837, 33, 5), (50_000, 1, 1)])
 def test_split_reduction_product_equals_plain_transposed_product(n, r, k):
     """tall_t_matmul splits the n-reduction into row chunks (and a tail when
-    n is not a multiple); it is the same product as A^T B, and for FP64
-    input falls back to it exactly."""
+    n is not a multiple); it is the same product as A^T B in FP32 and in
+    FP64."""
     from coherit_amd.precond import tall_t_matmul
 
     rng = np.random.default_rng(n + r + k)
@@ -146,4 +146,6 @@ def test_split_reduction_product_equals_plain_transposed_product(n, r, k):
     assert got.shape == (r, k) and got.dtype == torch.float32
     torch.testing.assert_close(got.double(), want, rtol=0, atol=2e-5 * float(np.sqrt(n)))
     A64, B64 = A.double(), B.double()
-    assert torch.equal(tall_t_matmul(A64, B64), A64.T @ B64)
+    got64 = tall_t_matmul(A64, B64)
+    assert got64.dtype == torch.float64
+    torch.testing.assert_close(got64, want, rtol=0, atol=1e-12 * float(np.sqrt(n)))
