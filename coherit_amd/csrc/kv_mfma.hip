@@ -104,10 +104,32 @@ __global__ __launch_bounds__(256) void k_colmax_part(int64_t rows, int64_t k,
   double w = 0.0;
   const float* mean = with_mean ? ss.mean + ss.set_of(c0 + c) * ss.stride + mean_off : nullptr;
   if (r < R) {
-    for (int64_t i = (int64_t)blockIdx.x * R + r; i < rows; i += (int64_t)gridDim.x * R) {
+    // rows i0, i0 + step, ... in that order; loads are issued kU rows ahead
+    // of the (order-preserving) max and FP64 sum chains
+    constexpr int kU = 8;
+    const int64_t step = (int64_t)gridDim.x * R;
+    int64_t i = (int64_t)blockIdx.x * R + r;
+    for (; i + (kU - 1) * step < rows; i += kU * step) {
+      float xs[kU], ms[kU];
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        xs[u] = X[(i + u * step) * ldx + c0 + c];
+        ms[u] = mean ? mean[i + u * step] : 0.0f;
+      }
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        // a non-finite entry counts as +inf (fmaxf would drop a NaN), so its
+        // column's scale becomes NaN and the column's outputs are NaN
+        mx = fmaxf(mx, isfinite(xs[u]) ? fabsf(xs[u]) : INFINITY);
+        if (mean) {
+          const float mux = ms[u] * xs[u];
+          mx = fmaxf(mx, isfinite(mux) ? fabsf(mux) : INFINITY);
+          w += static_cast<double>(ms[u]) * static_cast<double>(xs[u]);
+        }
+      }
+    }
+    for (; i < rows; i += step) {
       const float x = X[i * ldx + c0 + c];
-      // a non-finite entry counts as +inf (fmaxf would drop a NaN), so its
-      // column's scale becomes NaN and the column's outputs are NaN
       mx = fmaxf(mx, isfinite(x) ? fabsf(x) : INFINITY);
       if (mean) {
         const float mu = mean[i];

@@ -22,6 +22,7 @@ namespace coh {
 constexpr int kEPT = 8;               // elements per thread
 constexpr int kChunk = 256 * kEPT;    // elements per block
 constexpr int kJT = 8;                // basis vectors per register tile
+constexpr int kRedAll = 128;          // vectors whose partials fit LDS at once
 
 __device__ inline double wave_sum(double v) {
 #pragma unroll
@@ -29,11 +30,11 @@ __device__ inline double wave_sum(double v) {
   return v;
 }
 
-// part[(s * r + j) * nch + ch] = sum over chunk ch of P[s][j][i] X[s][i]
-__global__ __launch_bounds__(256) void k_probe_dots_part(int64_t n, int r, const float* __restrict__ P,
-                                                         int64_t ps, int64_t vs,
-                                                         const float* __restrict__ X, int64_t ldx,
-                                                         double* __restrict__ part, int nch) {
+// Body of k_probe_dots_part; FULL: the whole chunk lies inside [0, n).
+template <bool FULL>
+__device__ inline void probe_dots_body(int64_t n, int r, const float* __restrict__ P,
+                                       int64_t ps, int64_t vs, const float* __restrict__ X,
+                                       int64_t ldx, double* __restrict__ part, int nch) {
   const int s = blockIdx.y, ch = blockIdx.x, t = threadIdx.x;
   const int wave = t / 64, lane = t % 64;
   const int64_t i0 = (int64_t)ch * kChunk + t;
@@ -41,9 +42,37 @@ __global__ __launch_bounds__(256) void k_probe_dots_part(int64_t n, int r, const
 #pragma unroll
   for (int e = 0; e < kEPT; ++e) {
     const int64_t i = i0 + 256 * e;
-    x[e] = (i < n) ? X[(int64_t)s * ldx + i] : 0.0f;
+    x[e] = (FULL || i < n) ? X[(int64_t)s * ldx + i] : 0.0f;
   }
   const float* Ps = P + (int64_t)s * ps;
+  if (r <= kRedAll) {
+    // every vector's wave partials go to LDS; one barrier at the end (the
+    // same per-wave and four-wave sums as the tiled loop below)
+    __shared__ double all[kRedAll][4];
+    for (int j0 = 0; j0 < r; j0 += kJT) {
+      double acc[kJT];
+#pragma unroll
+      for (int jj = 0; jj < kJT; ++jj) {
+        acc[jj] = 0.0;
+        const int j = j0 + jj;
+        if (j < r) {
+#pragma unroll
+          for (int e = 0; e < kEPT; ++e) {
+            const int64_t i = i0 + 256 * e;
+            if (FULL || i < n) acc[jj] += static_cast<double>(Ps[(int64_t)j * vs + i]) * static_cast<double>(x[e]);
+          }
+        }
+      }
+#pragma unroll
+      for (int jj = 0; jj < kJT; ++jj) {
+        const double v = wave_sum(acc[jj]);
+        if (lane == 0 && j0 + jj < r) all[j0 + jj][wave] = v;
+      }
+    }
+    __syncthreads();
+    if (t < r) part[((int64_t)s * r + t) * nch + ch] = all[t][0] + all[t][1] + all[t][2] + all[t][3];
+    return;
+  }
   __shared__ double red[kJT][4];
   for (int j0 = 0; j0 < r; j0 += kJT) {
     double acc[kJT];
@@ -55,7 +84,7 @@ __global__ __launch_bounds__(256) void k_probe_dots_part(int64_t n, int r, const
 #pragma unroll
         for (int e = 0; e < kEPT; ++e) {
           const int64_t i = i0 + 256 * e;
-          if (i < n) acc[jj] += static_cast<double>(Ps[(int64_t)j * vs + i]) * static_cast<double>(x[e]);
+          if (FULL || i < n) acc[jj] += static_cast<double>(Ps[(int64_t)j * vs + i]) * static_cast<double>(x[e]);
         }
       }
     }
@@ -69,6 +98,18 @@ __global__ __launch_bounds__(256) void k_probe_dots_part(int64_t n, int r, const
       part[((int64_t)s * r + j0 + t) * nch + ch] = red[t][0] + red[t][1] + red[t][2] + red[t][3];
     __syncthreads();
   }
+}
+
+// part[(s * r + j) * nch + ch] = sum over chunk ch of P[s][j][i] X[s][i];
+// chunks wholly inside [0, n) take the body without per-element checks
+__global__ __launch_bounds__(256) void k_probe_dots_part(int64_t n, int r, const float* __restrict__ P,
+                                                         int64_t ps, int64_t vs,
+                                                         const float* __restrict__ X, int64_t ldx,
+                                                         double* __restrict__ part, int nch) {
+  if ((int64_t)(blockIdx.x + 1) * kChunk <= n)
+    probe_dots_body<true>(n, r, P, ps, vs, X, ldx, part, nch);
+  else
+    probe_dots_body<false>(n, r, P, ps, vs, X, ldx, part, nch);
 }
 
 __global__ void k_probe_dots_final(int64_t S, int r, int nch, const double* __restrict__ part,
@@ -85,11 +126,11 @@ __global__ void k_probe_dots_final(int64_t S, int r, int nch, const double* __re
 }
 
 // Y[s][i] = X[s][i] - sum_j P[s][j][i] C[s][j]; Y may alias X.
-__global__ __launch_bounds__(256) void k_probe_project_out(int64_t n, int r, const float* __restrict__ P,
-                                                           int64_t ps, int64_t vs,
-                                                           const float* __restrict__ C, int64_t ldc,
-                                                           const float* X, int64_t ldx, float* Y,
-                                                           int64_t ldy) {
+template <bool FULL>
+__device__ inline void probe_project_body(int64_t n, int r, const float* __restrict__ P,
+                                          int64_t ps, int64_t vs, const float* __restrict__ C,
+                                          int64_t ldc, const float* X, int64_t ldx, float* Y,
+                                          int64_t ldy) {
   const int s = blockIdx.y;
   const int64_t i0 = (int64_t)blockIdx.x * kChunk + threadIdx.x;
   const float* Ps = P + (int64_t)s * ps;
@@ -97,29 +138,40 @@ __global__ __launch_bounds__(256) void k_probe_project_out(int64_t n, int r, con
   float acc[kEPT];
 #pragma unroll
   for (int e = 0; e < kEPT; ++e) acc[e] = 0.0f;
+#pragma unroll 4
   for (int j = 0; j < r; ++j) {
     const float cj = Cs[j];
     const float* Pj = Ps + (int64_t)j * vs;
 #pragma unroll
     for (int e = 0; e < kEPT; ++e) {
       const int64_t i = i0 + 256 * e;
-      if (i < n) acc[e] = fmaf(Pj[i], cj, acc[e]);
+      if (FULL || i < n) acc[e] = fmaf(Pj[i], cj, acc[e]);
     }
   }
 #pragma unroll
   for (int e = 0; e < kEPT; ++e) {
     const int64_t i = i0 + 256 * e;
-    if (i < n) Y[(int64_t)s * ldy + i] = X[(int64_t)s * ldx + i] - acc[e];
+    if (FULL || i < n) Y[(int64_t)s * ldy + i] = X[(int64_t)s * ldx + i] - acc[e];
   }
 }
 
+__global__ __launch_bounds__(256) void k_probe_project_out(int64_t n, int r, const float* __restrict__ P,
+                                                           int64_t ps, int64_t vs,
+                                                           const float* __restrict__ C, int64_t ldc,
+                                                           const float* X, int64_t ldx, float* Y,
+                                                           int64_t ldy) {
+  if ((int64_t)(blockIdx.x + 1) * kChunk <= n)
+    probe_project_body<true>(n, r, P, ps, vs, C, ldc, X, ldx, Y, ldy);
+  else
+    probe_project_body<false>(n, r, P, ps, vs, C, ldc, X, ldx, Y, ldy);
+}
+
 // B[s][j][i] += sum_q coeff[s][j][q] V[s][q][i] for j < r, q < Q <= 4.
-__global__ __launch_bounds__(256) void k_probe_rank_update(int64_t n, int r, int Q,
-                                                           const float* __restrict__ coeff,
-                                                           int64_t cps, int64_t cvs,
-                                                           const float* __restrict__ V, int64_t vps,
-                                                           int64_t vvs, float* __restrict__ B,
-                                                           int64_t bps, int64_t bvs) {
+template <bool FULL>
+__device__ inline void probe_rank_body(int64_t n, int r, int Q, const float* __restrict__ coeff,
+                                       int64_t cps, int64_t cvs, const float* __restrict__ V,
+                                       int64_t vps, int64_t vvs, float* __restrict__ B,
+                                       int64_t bps, int64_t bvs) {
   const int s = blockIdx.y;
   const int64_t i0 = (int64_t)blockIdx.x * kChunk + threadIdx.x;
   float v[4][kEPT];
@@ -128,10 +180,11 @@ __global__ __launch_bounds__(256) void k_probe_rank_update(int64_t n, int r, int
 #pragma unroll
     for (int e = 0; e < kEPT; ++e) {
       const int64_t i = i0 + 256 * e;
-      v[q][e] = (q < Q && i < n) ? V[(int64_t)s * vps + (int64_t)q * vvs + i] : 0.0f;
+      v[q][e] = (q < Q && (FULL || i < n)) ? V[(int64_t)s * vps + (int64_t)q * vvs + i] : 0.0f;
     }
   float* Bs = B + (int64_t)s * bps;
   const float* Cs = coeff + (int64_t)s * cps;
+#pragma unroll 4
   for (int j = 0; j < r; ++j) {
     float c[4];
 #pragma unroll
@@ -140,7 +193,7 @@ __global__ __launch_bounds__(256) void k_probe_rank_update(int64_t n, int r, int
 #pragma unroll
     for (int e = 0; e < kEPT; ++e) {
       const int64_t i = i0 + 256 * e;
-      if (i < n) {
+      if (FULL || i < n) {
         float add = 0.0f;
 #pragma unroll
         for (int q = 0; q < 4; ++q) add = fmaf(c[q], v[q][e], add);
@@ -148,6 +201,18 @@ __global__ __launch_bounds__(256) void k_probe_rank_update(int64_t n, int r, int
       }
     }
   }
+}
+
+__global__ __launch_bounds__(256) void k_probe_rank_update(int64_t n, int r, int Q,
+                                                           const float* __restrict__ coeff,
+                                                           int64_t cps, int64_t cvs,
+                                                           const float* __restrict__ V, int64_t vps,
+                                                           int64_t vvs, float* __restrict__ B,
+                                                           int64_t bps, int64_t bvs) {
+  if ((int64_t)(blockIdx.x + 1) * kChunk <= n)
+    probe_rank_body<true>(n, r, Q, coeff, cps, cvs, V, vps, vvs, B, bps, bvs);
+  else
+    probe_rank_body<false>(n, r, Q, coeff, cps, cvs, V, vps, vvs, B, bps, bvs);
 }
 
 static int check_dims(int64_t S, int64_t r, int64_t n) {
