@@ -105,6 +105,26 @@ def measured_traffic(kernel: str):
         return None
 
 
+def measured_copy_gbps(nbytes: int = 2 << 30, reps: int = 5) -> float:
+    """Device-to-device copy rate (bytes read + written per second, GB/s):
+    the practical HBM ceiling on this box, next to the published peak."""
+    import torch
+
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    stop.record()
+    stop.synchronize()
+    sec = start.elapsed_time(stop) / 1e3 / reps
+    del src, dst
+    torch.cuda.empty_cache()
+    return 2.0 * nbytes / sec / 1e9
+
+
 def roofline(timing: dict):
     """Dominant kernel (largest event time) and its roofline numbers."""
     if not timing:
@@ -511,6 +531,7 @@ def main():
     for _ in range(args.warmup):
         one_fit()
     torch.cuda.synchronize()
+    copy_gbps = measured_copy_gbps()
     P.CALL_LOG = []
     N.timing_reset()
     N.timing_enable(True)
@@ -530,6 +551,11 @@ def main():
     ms_per_step = elapsed / args.steps * 1e3
     fits_per_hour = 3600.0 / (elapsed / args.steps)
     name, roof = roofline(timing)
+    if roof is not None:
+        # the measured ceiling beside the published peak
+        roof["measured_copy_GBps"] = round(copy_gbps, 1)
+        if roof["bound"] == "hbm":
+            roof["frac_of_measured_copy"] = round(roof["achieved"] / copy_gbps, 4)
     # per fit (the call log spans all timed fits)
     kv_calls = sum(1 for c in call_log if c[0] == "kv") // max(1, args.steps)
     log(args, f"fit: {ms_per_step / 1e3:.1f}s  h2={res.heritability:.4f} (true {wl.h2.sum():.2f}) "
