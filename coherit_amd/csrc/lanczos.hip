@@ -222,6 +222,213 @@ static int check_dims(int64_t S, int64_t r, int64_t n) {
   return COH_OK;
 }
 
+// ----------------------------------------------------- fused Lanczos step
+//
+// One re-orthogonalised Lanczos step (slq.py:49-74) around the two passes
+// above, with the step's reductions in FP64 partials:
+//   prep:   w0 = product - beta_prev * prev, alpha = q . w0, w1 = w0 - alpha q,
+//           active = (q . q > 0), scale = |product|
+//   (two re-orthogonalisation passes: w1 -> w2 -> w3)
+//   finish: square = |w3|^2, norm = sqrt(square > 0 ? square : 1),
+//           keep = active and square > (16 eps max(scale, |alpha|))^2,
+//           beta = keep ? norm : 0, alpha_out = active ? alpha : 1,
+//           next = keep ? w3 / norm : 0
+// q = basis[s][index], prev = basis[s][index - 1] (zero at index 0).  The
+// FP32 element-wise expressions are the reference's, without contraction.
+
+constexpr float kEps32 = 1.1920928955078125e-07f;
+
+// Sum of one double per thread over the block: result valid in thread 0.
+__device__ inline void block_sums3(double a, double b, double c, double* out3) {
+  __shared__ double red[3][4];
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  a = wave_sum(a);
+  b = wave_sum(b);
+  c = wave_sum(c);
+  if (lane == 0) {
+    red[0][wave] = a;
+    red[1][wave] = b;
+    red[2][wave] = c;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int v = 0; v < 3; ++v) out3[v] = red[v][0] + red[v][1] + red[v][2] + red[v][3];
+  }
+}
+
+template <bool FULL>
+__device__ inline void lz_prep_part_body(int64_t n, int64_t index, const float* __restrict__ basis,
+                                         int64_t ps, int64_t vs, const float* __restrict__ product,
+                                         int64_t ldp, const float* __restrict__ prev_beta,
+                                         double* __restrict__ part, int nch) {
+#pragma clang fp contract(off)
+  const int s = blockIdx.y, ch = blockIdx.x;
+  const int64_t i0 = (int64_t)ch * kChunk + threadIdx.x;
+  const float* q = basis + (int64_t)s * ps + index * vs;
+  const float* pv = index > 0 ? q - vs : nullptr;
+  const float* pr = product + (int64_t)s * ldp;
+  const float b = prev_beta[s];
+  double qq = 0.0, qw = 0.0, pp = 0.0;
+#pragma unroll
+  for (int e = 0; e < kEPT; ++e) {
+    const int64_t i = i0 + 256 * e;
+    if (FULL || i < n) {
+      const float qi = q[i], pi = pr[i];
+      const float w0 = pv ? pi - pv[i] * b : pi;
+      qq += static_cast<double>(qi) * static_cast<double>(qi);
+      qw += static_cast<double>(qi) * static_cast<double>(w0);
+      pp += static_cast<double>(pi) * static_cast<double>(pi);
+    }
+  }
+  double sums[3];
+  block_sums3(qq, qw, pp, sums);
+  if (threadIdx.x == 0) {
+    double* dst = part + ((int64_t)s * nch + ch) * 3;
+    dst[0] = sums[0];
+    dst[1] = sums[1];
+    dst[2] = sums[2];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_lz_prep_part(int64_t n, int64_t index,
+                                                      const float* __restrict__ basis, int64_t ps,
+                                                      int64_t vs, const float* __restrict__ product,
+                                                      int64_t ldp, const float* __restrict__ prev_beta,
+                                                      double* __restrict__ part, int nch) {
+  if ((int64_t)(blockIdx.x + 1) * kChunk <= n)
+    lz_prep_part_body<true>(n, index, basis, ps, vs, product, ldp, prev_beta, part, nch);
+  else
+    lz_prep_part_body<false>(n, index, basis, ps, vs, product, ldp, prev_beta, part, nch);
+}
+
+// One thread per probe: chunk partials summed in chunk order.
+__global__ void k_lz_prep_final(int64_t S, int nch, const double* __restrict__ part,
+                                float* __restrict__ alpha, uint8_t* __restrict__ active,
+                                float* __restrict__ scale) {
+  const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (s >= S) return;
+  double qq = 0.0, qw = 0.0, pp = 0.0;
+  for (int ch = 0; ch < nch; ++ch) {
+    const double* p = part + (s * nch + ch) * 3;
+    qq += p[0];
+    qw += p[1];
+    pp += p[2];
+  }
+  alpha[s] = static_cast<float>(qw);
+  active[s] = qq > 0.0 ? 1 : 0;
+  scale[s] = sqrtf(static_cast<float>(pp));
+}
+
+template <bool FULL>
+__device__ inline void lz_prep_w_body(int64_t n, int64_t index, const float* __restrict__ basis,
+                                      int64_t ps, int64_t vs, const float* __restrict__ product,
+                                      int64_t ldp, const float* __restrict__ prev_beta,
+                                      const float* __restrict__ alpha, float* __restrict__ w0,
+                                      float* __restrict__ w1, int64_t ldw) {
+#pragma clang fp contract(off)
+  const int s = blockIdx.y;
+  const int64_t i0 = (int64_t)blockIdx.x * kChunk + threadIdx.x;
+  const float* q = basis + (int64_t)s * ps + index * vs;
+  const float* pv = index > 0 ? q - vs : nullptr;
+  const float* pr = product + (int64_t)s * ldp;
+  const float b = prev_beta[s], a = alpha[s];
+#pragma unroll
+  for (int e = 0; e < kEPT; ++e) {
+    const int64_t i = i0 + 256 * e;
+    if (FULL || i < n) {
+      const float v0 = pv ? pr[i] - pv[i] * b : pr[i];
+      w0[(int64_t)s * ldw + i] = v0;
+      w1[(int64_t)s * ldw + i] = v0 - q[i] * a;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_lz_prep_w(int64_t n, int64_t index,
+                                                   const float* __restrict__ basis, int64_t ps,
+                                                   int64_t vs, const float* __restrict__ product,
+                                                   int64_t ldp, const float* __restrict__ prev_beta,
+                                                   const float* __restrict__ alpha,
+                                                   float* __restrict__ w0, float* __restrict__ w1,
+                                                   int64_t ldw) {
+  if ((int64_t)(blockIdx.x + 1) * kChunk <= n)
+    lz_prep_w_body<true>(n, index, basis, ps, vs, product, ldp, prev_beta, alpha, w0, w1, ldw);
+  else
+    lz_prep_w_body<false>(n, index, basis, ps, vs, product, ldp, prev_beta, alpha, w0, w1, ldw);
+}
+
+template <bool FULL>
+__device__ inline void lz_sq_part_body(int64_t n, const float* __restrict__ w3, int64_t ldw,
+                                       double* __restrict__ part, int nch) {
+  const int s = blockIdx.y, ch = blockIdx.x;
+  const int64_t i0 = (int64_t)ch * kChunk + threadIdx.x;
+  double sq = 0.0;
+#pragma unroll
+  for (int e = 0; e < kEPT; ++e) {
+    const int64_t i = i0 + 256 * e;
+    if (FULL || i < n) {
+      const double v = static_cast<double>(w3[(int64_t)s * ldw + i]);
+      sq += v * v;
+    }
+  }
+  double sums[3];
+  block_sums3(sq, 0.0, 0.0, sums);
+  if (threadIdx.x == 0) part[(int64_t)s * nch + ch] = sums[0];
+}
+
+__global__ __launch_bounds__(256) void k_lz_sq_part(int64_t n, const float* __restrict__ w3,
+                                                    int64_t ldw, double* __restrict__ part, int nch) {
+  if ((int64_t)(blockIdx.x + 1) * kChunk <= n)
+    lz_sq_part_body<true>(n, w3, ldw, part, nch);
+  else
+    lz_sq_part_body<false>(n, w3, ldw, part, nch);
+}
+
+__global__ void k_lz_finish_final(int64_t S, int nch, const double* __restrict__ part,
+                                  const float* __restrict__ alpha, const uint8_t* __restrict__ active,
+                                  const float* __restrict__ scale, float* __restrict__ norm,
+                                  uint8_t* __restrict__ keep, float* __restrict__ beta,
+                                  float* __restrict__ alpha_out) {
+#pragma clang fp contract(off)
+  const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (s >= S) return;
+  double sq = 0.0;
+  for (int ch = 0; ch < nch; ++ch) sq += part[s * nch + ch];
+  const float square = static_cast<float>(sq);
+  const float nm = sqrtf(square > 0.0f ? square : 1.0f);
+  const float threshold = 16.0f * kEps32 * fmaxf(scale[s], fabsf(alpha[s]));
+  const bool k = active[s] && square > threshold * threshold;
+  norm[s] = nm;
+  keep[s] = k ? 1 : 0;
+  if (beta) beta[s] = k ? nm : 0.0f;
+  if (alpha_out) alpha_out[s] = active[s] ? alpha[s] : 1.0f;
+}
+
+template <bool FULL>
+__device__ inline void lz_next_body(int64_t n, const float* __restrict__ w3, int64_t ldw,
+                                    const float* __restrict__ norm, const uint8_t* __restrict__ keep,
+                                    float* __restrict__ next, int64_t ldn) {
+  const int s = blockIdx.y;
+  const int64_t i0 = (int64_t)blockIdx.x * kChunk + threadIdx.x;
+  const bool k = keep[s] != 0;
+  const float nm = norm[s];
+#pragma unroll
+  for (int e = 0; e < kEPT; ++e) {
+    const int64_t i = i0 + 256 * e;
+    if (FULL || i < n) next[(int64_t)s * ldn + i] = k ? w3[(int64_t)s * ldw + i] / nm : 0.0f;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_lz_next(int64_t n, const float* __restrict__ w3,
+                                                 int64_t ldw, const float* __restrict__ norm,
+                                                 const uint8_t* __restrict__ keep,
+                                                 float* __restrict__ next, int64_t ldn) {
+  if ((int64_t)(blockIdx.x + 1) * kChunk <= n)
+    lz_next_body<true>(n, w3, ldw, norm, keep, next, ldn);
+  else
+    lz_next_body<false>(n, w3, ldw, norm, keep, next, ldn);
+}
+
 }  // namespace coh
 
 using namespace coh;
@@ -295,5 +502,75 @@ extern "C" int coh_probe_rank_update(int64_t S, int64_t r, int64_t n, int32_t q,
                      static_cast<hipStream_t>(stream), n, (int)r, (int)q, coeff, cps, cvs, V, vps,
                      vvs, B, bps, bvs);
   COH_CHECK_LAUNCH();
+  return COH_OK;
+}
+
+extern "C" int coh_probe_step_workspace_bytes(int64_t S, int64_t n, uint64_t* bytes) {
+  COH_REQUIRE(bytes != nullptr, COH_ERR_VALUE, "bytes is NULL");
+  int st = check_dims(S, 0, n);
+  if (st != COH_OK) return st;
+  const int64_t nch = std::max<int64_t>(1, ceil_div(n, kChunk));
+  *bytes = (uint64_t)round_up(std::max<int64_t>(1, S * nch * 3) * 8, 256);
+  return COH_OK;
+}
+
+extern "C" int coh_probe_step_prep(int64_t S, int64_t n, int64_t index, const float* basis,
+                                int64_t ps, int64_t vs, const float* product, int64_t ldp,
+                                const float* prev_beta, float* w0, float* w1, int64_t ldw,
+                                float* alpha, uint8_t* active, float* scale, void* ws,
+                                uint64_t ws_bytes, void* stream) {
+  int st = check_dims(S, 0, n);
+  if (st != COH_OK) return st;
+  if (S == 0) return COH_OK;
+  COH_REQUIRE(basis && product && prev_beta && w0 && w1 && alpha && active && scale && ws,
+              COH_ERR_VALUE, "NULL pointer argument");
+  COH_REQUIRE(index >= 0 && vs >= n && ps >= (index + 1) * vs && ldp >= n && ldw >= n,
+              COH_ERR_VALUE, "inconsistent strides");
+  uint64_t need = 0;
+  coh_probe_step_workspace_bytes(S, n, &need);
+  COH_REQUIRE(ws_bytes >= need, COH_ERR_VALUE, "workspace too small");
+  const int nch = (int)std::max<int64_t>(1, ceil_div(n, kChunk));
+  hipStream_t hs = static_cast<hipStream_t>(stream);
+  double* part = static_cast<double*>(ws);
+  hipLaunchKernelGGL(k_lz_prep_part, dim3(nch, (unsigned)S), dim3(256), 0, hs, n, index, basis,
+                     ps, vs, product, ldp, prev_beta, part, nch);
+  COH_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_lz_prep_final, dim3((unsigned)ceil_div(S, 64)), dim3(64), 0, hs, S, nch,
+                     part, alpha, active, scale);
+  COH_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_lz_prep_w, dim3(nch, (unsigned)S), dim3(256), 0, hs, n, index, basis, ps,
+                     vs, product, ldp, prev_beta, alpha, w0, w1, ldw);
+  COH_CHECK_LAUNCH();
+  return COH_OK;
+}
+
+extern "C" int coh_probe_step_finish(int64_t S, int64_t n, const float* w3, int64_t ldw,
+                                  const float* alpha, const uint8_t* active, const float* scale,
+                                  float* norm, uint8_t* keep, float* beta, float* alpha_out,
+                                  float* next, int64_t ldn, void* ws, uint64_t ws_bytes,
+                                  void* stream) {
+  int st = check_dims(S, 0, n);
+  if (st != COH_OK) return st;
+  if (S == 0) return COH_OK;
+  COH_REQUIRE(w3 && alpha && active && scale && norm && keep && ws, COH_ERR_VALUE,
+              "NULL pointer argument");
+  COH_REQUIRE(ldw >= n && (next == nullptr || ldn >= n), COH_ERR_VALUE, "inconsistent strides");
+  uint64_t need = 0;
+  coh_probe_step_workspace_bytes(S, n, &need);
+  COH_REQUIRE(ws_bytes >= need, COH_ERR_VALUE, "workspace too small");
+  const int nch = (int)std::max<int64_t>(1, ceil_div(n, kChunk));
+  hipStream_t hs = static_cast<hipStream_t>(stream);
+  double* part = static_cast<double*>(ws);
+  hipLaunchKernelGGL(k_lz_sq_part, dim3(nch, (unsigned)S), dim3(256), 0, hs, n, w3, ldw, part,
+                     nch);
+  COH_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_lz_finish_final, dim3((unsigned)ceil_div(S, 64)), dim3(64), 0, hs, S, nch,
+                     part, alpha, active, scale, norm, keep, beta, alpha_out);
+  COH_CHECK_LAUNCH();
+  if (next) {
+    hipLaunchKernelGGL(k_lz_next, dim3(nch, (unsigned)S), dim3(256), 0, hs, n, w3, ldw, norm,
+                       keep, next, ldn);
+    COH_CHECK_LAUNCH();
+  }
   return COH_OK;
 }

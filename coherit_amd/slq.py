@@ -111,7 +111,78 @@ def _reorth(P, w):
     return c1, w2, c2, w3
 
 
+# Partial-sum workspace of coh_probe_step_prep/finish, one per host thread.
+_STEP_LOCAL = threading.local()
+
+
+def _rows(t):
+    """Row stride of a (rows, n) tensor with unit element stride; a single
+    row may carry any stride (torch keeps the transposed stride of a
+    one-row transpose), and then only row 0 is read."""
+    return t.stride(0) if t.shape[0] > 1 else t.shape[-1]
+
+
+def _step_parts_device(basis, product, previous_beta, index, next_out=None):
+    """The quantities of :func:`_step_parts` from the fused step kernels
+    (coh_probe_step_prep, the two re-orthogonalisation passes,
+    coh_probe_step_finish): 12 launches where the element-wise torch form
+    takes about 45.  With ``next_out`` (a (S, n) view with unit element
+    stride) the next Lanczos vector is written there and ``beta`` and
+    ``alpha_out`` are returned too."""
+    S, _, n = basis.shape
+    dev = basis.device
+    if basis.stride(2) != 1:
+        basis = basis.contiguous()
+    if product.stride(1) != 1:
+        product = product.contiguous()
+    pb = previous_beta.to(device=dev, dtype=torch.float32).contiguous()
+    nbytes = ctypes.c_uint64()
+    N.call("coh_probe_step_workspace_bytes", S, n, ctypes.byref(nbytes))
+    key = (dev, int(nbytes.value))
+    ws = getattr(_STEP_LOCAL, "ws", {}).get(key)
+    if ws is None:
+        ws = torch.empty(max(int(nbytes.value), 256), dtype=torch.uint8, device=dev)
+        _STEP_LOCAL.ws = {key: ws}
+    w0 = torch.empty((S, n), dtype=torch.float32, device=dev)
+    w1 = torch.empty_like(w0)
+    alpha = torch.empty(S, dtype=torch.float32, device=dev)
+    active = torch.empty(S, dtype=torch.bool, device=dev)
+    scale = torch.empty(S, dtype=torch.float32, device=dev)
+    N.call("coh_probe_step_prep", S, n, int(index), N.ptr(basis),
+           basis.stride(0) if S > 1 else basis.shape[1] * basis.stride(1), basis.stride(1),
+           N.ptr(product), _rows(product), N.ptr(pb), N.ptr(w0), N.ptr(w1), _rows(w0),
+           N.ptr(alpha), N.ptr(active), N.ptr(scale), N.ptr(ws), int(nbytes.value),
+           N.stream_ptr())
+    P = basis[:, : index + 1, :]
+    c1, w2, c2, w3 = _reorth(P, w1)
+    if w3.stride(1) != 1:
+        w3 = w3.contiguous()
+    norm = torch.empty(S, dtype=torch.float32, device=dev)
+    keep = torch.empty(S, dtype=torch.bool, device=dev)
+    beta = alpha_out = None
+    if next_out is not None:
+        if next_out.stride(1) != 1:
+            raise ValueError("next_out needs unit element stride.")
+        beta = torch.empty(S, dtype=torch.float32, device=dev)
+        alpha_out = torch.empty(S, dtype=torch.float32, device=dev)
+    N.call("coh_probe_step_finish", S, n, N.ptr(w3), _rows(w3), N.ptr(alpha), N.ptr(active),
+           N.ptr(scale), N.ptr(norm), N.ptr(keep), N.ptr(beta), N.ptr(alpha_out),
+           N.ptr(next_out), _rows(next_out) if next_out is not None else n, N.ptr(ws),
+           int(nbytes.value), N.stream_ptr())
+    q = basis[:, index, :]
+    prev = basis[:, index - 1, :] if index > 0 else torch.zeros_like(q)
+    return dict(q=q, prev=prev, active=active, w0=w0, alpha=alpha, w1=w1, P=P, c1=c1, w2=w2,
+                c2=c2, w3=w3, norm=norm, keep=keep, beta=beta, alpha_out=alpha_out)
+
+
 def _step_parts(basis, product, previous_beta, index):
+    if _on_device(basis, product):
+        return _step_parts_device(basis, product, previous_beta, index)
+    return _step_parts_torch(basis, product, previous_beta, index)
+
+
+def _step_parts_torch(basis, product, previous_beta, index):
+    """The step's quantities as element-wise torch operations (slq.py:49-74)."""
     q = basis[:, index, :]
     prev = basis[:, index - 1, :] if index > 0 else torch.zeros_like(q)
     active = (q * q).sum(1) > 0
@@ -130,7 +201,12 @@ def _step_parts(basis, product, previous_beta, index):
 
 
 def _step(basis, product, previous_beta, index):
-    """Two-pass orthogonalised Lanczos step (slq.py:49-74)."""
+    """Two-pass orthogonalised Lanczos step (slq.py:49-74).  On the device the
+    next vector is written straight into ``basis[:, index + 1]``."""
+    if _on_device(basis, product) and basis.stride(2) == 1:
+        nxt = basis[:, index + 1, :]
+        s = _step_parts_device(basis, product, previous_beta, index, next_out=nxt)
+        return nxt, s["alpha_out"], s["beta"]
     s = _step_parts(basis, product, previous_beta, index)
     keep = s["keep"]
     beta = torch.where(keep, s["norm"], torch.zeros_like(s["norm"]))
@@ -278,7 +354,8 @@ def logdet_value_and_grad(matvec, matvec_pullback, n, key, *, nsamples, m,
             nq, a, b = _step(basis, prod, prev_beta, index)
             if products is not None:
                 products[:, index, :] = prod
-            basis[:, index + 1, :] = nq
+            if nq.data_ptr() != basis[:, index + 1, :].data_ptr():
+                basis[:, index + 1, :] = nq
             alphas.append(a)
             betas.append(b)
             prev_beta = b

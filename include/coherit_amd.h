@@ -279,6 +279,30 @@ int coh_probe_project_out(int64_t S, int64_t r, int64_t n, const float* dev_P, i
 int coh_probe_rank_update(int64_t S, int64_t r, int64_t n, int32_t Q, const float* dev_coeff,
                           int64_t cps, int64_t cvs, const float* dev_V, int64_t vps, int64_t vvs,
                           float* dev_B, int64_t bps, int64_t bvs, void* stream);
+/* One re-orthogonalised Lanczos step of S probes (slq.py:49-74, the
+ * reference's _lanczos_step), around the two re-orthogonalisation passes
+ * above.  q = basis[s][index], prev = basis[s][index-1] (zero at index 0),
+ * basis probe-major with probe stride ps and vector stride vs; product
+ * (S rows, stride ldp) = operator applied to q.  Reductions accumulate FP32
+ * products in FP64 in a fixed order.
+ *   prep:   w0 = product - prev_beta * prev, alpha = q . w0,
+ *           w1 = w0 - alpha q, active = (q . q > 0), scale = |product|
+ *   finish: square = |w3|^2, norm = sqrt(square > 0 ? square : 1),
+ *           keep = active and square > (16 eps32 max(scale, |alpha|))^2,
+ *           beta = keep ? norm : 0, alpha_out = active ? alpha : 1, and
+ *           when next != NULL, next = keep ? w3 / norm : 0 (slq.py:66-74).
+ * active and keep are 0/1 bytes; beta and alpha_out may be NULL. */
+int coh_probe_step_workspace_bytes(int64_t S, int64_t n, uint64_t* bytes);
+int coh_probe_step_prep(int64_t S, int64_t n, int64_t index, const float* dev_basis, int64_t ps,
+                        int64_t vs, const float* dev_product, int64_t ldp,
+                        const float* dev_prev_beta, float* dev_w0, float* dev_w1, int64_t ldw,
+                        float* dev_alpha, uint8_t* dev_active, float* dev_scale,
+                        void* dev_workspace, uint64_t workspace_bytes, void* stream);
+int coh_probe_step_finish(int64_t S, int64_t n, const float* dev_w3, int64_t ldw,
+                          const float* dev_alpha, const uint8_t* dev_active,
+                          const float* dev_scale, float* dev_norm, uint8_t* dev_keep,
+                          float* dev_beta, float* dev_alpha_out, float* dev_next, int64_t ldn,
+                          void* dev_workspace, uint64_t workspace_bytes, void* stream);
 
 /* ---------------------------------------------------------------------
  * Kernel timing instrumentation (the reference only logs wall-clock
