@@ -429,6 +429,204 @@ __global__ __launch_bounds__(256) void k_lz_next(int64_t n, const float* __restr
     lz_next_body<false>(n, w3, ldw, norm, keep, next, ldn);
 }
 
+// ---- Adjoint of the step (slq.py:49-74 reversed) ----------------------
+// Head:  nxt = w3 / norm, proj = nxt . q_bar_next,
+//        w3_bar = keep ? (q_bar_next - nxt * proj) / norm + beta_bar * nxt : 0.
+// Tail:  a_tot = (active ? alpha_bar : 0) - q . w1_bar,
+//        w0_bar = w1_bar + q * a_tot,
+//        basis_bar[index]     += -alpha * w1_bar + a_tot * w0,
+//        basis_bar[index - 1] += -prev_beta * w0_bar,
+//        prev_beta_bar = -(prev . w0_bar).
+// Every block sums the chunk partials of its probe itself, in one fixed
+// order, so all blocks of a probe see the same total without a separate
+// launch.
+
+// Fixed-order sum of part[0..nch) by the whole block, returned to every thread.
+__device__ inline double block_total(const double* __restrict__ part, int nch) {
+  __shared__ double red[4];
+  __shared__ double total;
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  double v = 0.0;
+  for (int c = threadIdx.x; c < nch; c += 256) v += part[c];
+  v = wave_sum(v);
+  if (lane == 0) red[wave] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) total = red[0] + red[1] + red[2] + red[3];
+  __syncthreads();
+  return total;
+}
+
+template <bool FULL>
+__device__ inline void lz_adj_head_part_body(int64_t n, const float* __restrict__ w3, int64_t ldw,
+                                             const float* __restrict__ norm,
+                                             const float* __restrict__ qbn, int64_t ldq,
+                                             double* __restrict__ part, int nch) {
+#pragma clang fp contract(off)
+  const int s = blockIdx.y, ch = blockIdx.x;
+  const int64_t i0 = (int64_t)ch * kChunk + threadIdx.x;
+  const float nm = norm[s];
+  double d = 0.0;
+#pragma unroll
+  for (int e = 0; e < kEPT; ++e) {
+    const int64_t i = i0 + 256 * e;
+    if (FULL || i < n) {
+      const float nx = w3[(int64_t)s * ldw + i] / nm;
+      d += static_cast<double>(nx) * static_cast<double>(qbn[(int64_t)s * ldq + i]);
+    }
+  }
+  double sums[3];
+  block_sums3(d, 0.0, 0.0, sums);
+  if (threadIdx.x == 0) part[(int64_t)s * nch + ch] = sums[0];
+}
+
+__global__ __launch_bounds__(256) void k_lz_adj_head_part(int64_t n, const float* __restrict__ w3,
+                                                          int64_t ldw, const float* __restrict__ norm,
+                                                          const float* __restrict__ qbn, int64_t ldq,
+                                                          double* __restrict__ part, int nch) {
+  if ((int64_t)(blockIdx.x + 1) * kChunk <= n)
+    lz_adj_head_part_body<true>(n, w3, ldw, norm, qbn, ldq, part, nch);
+  else
+    lz_adj_head_part_body<false>(n, w3, ldw, norm, qbn, ldq, part, nch);
+}
+
+template <bool FULL>
+__device__ inline void lz_adj_head_w_body(int64_t n, const float* __restrict__ w3, int64_t ldw,
+                                          const float* __restrict__ norm,
+                                          const uint8_t* __restrict__ keep,
+                                          const float* __restrict__ qbn, int64_t ldq,
+                                          const float* __restrict__ beta_bar,
+                                          const double* __restrict__ part, int nch,
+                                          float* __restrict__ w3b, int64_t ldb) {
+#pragma clang fp contract(off)
+  const int s = blockIdx.y;
+  const int64_t i0 = (int64_t)blockIdx.x * kChunk + threadIdx.x;
+  const float proj = static_cast<float>(block_total(part + (int64_t)s * nch, nch));
+  const float nm = norm[s], bb = beta_bar[s];
+  const bool k = keep[s] != 0;
+#pragma unroll
+  for (int e = 0; e < kEPT; ++e) {
+    const int64_t i = i0 + 256 * e;
+    if (FULL || i < n) {
+      float v = 0.0f;
+      if (k) {
+        const float nx = w3[(int64_t)s * ldw + i] / nm;
+        const float t = (qbn[(int64_t)s * ldq + i] - nx * proj) / nm;
+        v = t + bb * nx;
+      }
+      w3b[(int64_t)s * ldb + i] = v;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_lz_adj_head_w(int64_t n, const float* __restrict__ w3,
+                                                       int64_t ldw, const float* __restrict__ norm,
+                                                       const uint8_t* __restrict__ keep,
+                                                       const float* __restrict__ qbn, int64_t ldq,
+                                                       const float* __restrict__ beta_bar,
+                                                       const double* __restrict__ part, int nch,
+                                                       float* __restrict__ w3b, int64_t ldb) {
+  if ((int64_t)(blockIdx.x + 1) * kChunk <= n)
+    lz_adj_head_w_body<true>(n, w3, ldw, norm, keep, qbn, ldq, beta_bar, part, nch, w3b, ldb);
+  else
+    lz_adj_head_w_body<false>(n, w3, ldw, norm, keep, qbn, ldq, beta_bar, part, nch, w3b, ldb);
+}
+
+template <bool FULL>
+__device__ inline void lz_adj_tail_part_body(int64_t n, const float* __restrict__ q, int64_t ps,
+                                             const float* __restrict__ w1b, int64_t ld1,
+                                             double* __restrict__ part, int nch) {
+  const int s = blockIdx.y, ch = blockIdx.x;
+  const int64_t i0 = (int64_t)ch * kChunk + threadIdx.x;
+  double d = 0.0;
+#pragma unroll
+  for (int e = 0; e < kEPT; ++e) {
+    const int64_t i = i0 + 256 * e;
+    if (FULL || i < n)
+      d += static_cast<double>(q[(int64_t)s * ps + i]) *
+           static_cast<double>(w1b[(int64_t)s * ld1 + i]);
+  }
+  double sums[3];
+  block_sums3(d, 0.0, 0.0, sums);
+  if (threadIdx.x == 0) part[(int64_t)s * nch + ch] = sums[0];
+}
+
+__global__ __launch_bounds__(256) void k_lz_adj_tail_part(int64_t n, const float* __restrict__ q,
+                                                          int64_t ps, const float* __restrict__ w1b,
+                                                          int64_t ld1, double* __restrict__ part,
+                                                          int nch) {
+  if ((int64_t)(blockIdx.x + 1) * kChunk <= n)
+    lz_adj_tail_part_body<true>(n, q, ps, w1b, ld1, part, nch);
+  else
+    lz_adj_tail_part_body<false>(n, q, ps, w1b, ld1, part, nch);
+}
+
+struct AdjTailArgs {
+  int64_t n, index, ps, vs, ld0, ld1, bps, bvs, ldo;
+  const float* basis;
+  const float* w0;
+  const float* w1b;
+  const float* alpha;
+  const uint8_t* active;
+  const float* alpha_bar;
+  const float* prev_beta;
+  float* basis_bar;
+  float* w0b;
+  const double* part_in;
+  double* part_out;
+  int nch;
+};
+
+template <bool FULL>
+__device__ inline void lz_adj_tail_w_body(const AdjTailArgs& a) {
+#pragma clang fp contract(off)
+  const int s = blockIdx.y, ch = blockIdx.x;
+  const int64_t i0 = (int64_t)ch * kChunk + threadIdx.x;
+  const float dot = static_cast<float>(block_total(a.part_in + (int64_t)s * a.nch, a.nch));
+  const float a_tot = (a.active[s] ? a.alpha_bar[s] : 0.0f) - dot;
+  const float neg_alpha = -a.alpha[s];
+  const float neg_pb = -a.prev_beta[s];
+  const float* q = a.basis + (int64_t)s * a.ps + a.index * a.vs;
+  const float* pv = a.index > 0 ? q - a.vs : nullptr;
+  float* bq = a.basis_bar + (int64_t)s * a.bps + a.index * a.bvs;
+  float* bp = a.index > 0 ? bq - a.bvs : nullptr;
+  double d = 0.0;
+#pragma unroll
+  for (int e = 0; e < kEPT; ++e) {
+    const int64_t i = i0 + 256 * e;
+    if (FULL || i < a.n) {
+      const float w1b = a.w1b[(int64_t)s * a.ld1 + i];
+      const float w0b = w1b + q[i] * a_tot;
+      a.w0b[(int64_t)s * a.ldo + i] = w0b;
+      const float t1 = neg_alpha * w1b;
+      const float t2 = a_tot * a.w0[(int64_t)s * a.ld0 + i];
+      bq[i] = bq[i] + (t1 + t2);
+      if (pv) {
+        bp[i] = bp[i] + neg_pb * w0b;
+        d += static_cast<double>(pv[i]) * static_cast<double>(w0b);
+      }
+    }
+  }
+  double sums[3];
+  block_sums3(d, 0.0, 0.0, sums);
+  if (threadIdx.x == 0) a.part_out[(int64_t)s * a.nch + ch] = sums[0];
+}
+
+__global__ __launch_bounds__(256) void k_lz_adj_tail_w(AdjTailArgs a) {
+  if ((int64_t)(blockIdx.x + 1) * kChunk <= a.n)
+    lz_adj_tail_w_body<true>(a);
+  else
+    lz_adj_tail_w_body<false>(a);
+}
+
+__global__ void k_lz_adj_tail_final(int64_t S, int nch, const double* __restrict__ part,
+                                    float* __restrict__ prev_beta_bar) {
+  const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (s >= S) return;
+  double d = 0.0;
+  for (int ch = 0; ch < nch; ++ch) d += part[s * nch + ch];
+  prev_beta_bar[s] = -static_cast<float>(d);
+}
+
 }  // namespace coh
 
 using namespace coh;
@@ -572,5 +770,68 @@ extern "C" int coh_probe_step_finish(int64_t S, int64_t n, const float* w3, int6
                        keep, next, ldn);
     COH_CHECK_LAUNCH();
   }
+  return COH_OK;
+}
+
+extern "C" int coh_probe_step_adjoint_head(int64_t S, int64_t n, const float* w3, int64_t ldw,
+                                          const float* norm, const uint8_t* keep,
+                                          const float* q_bar_next, int64_t ldq,
+                                          const float* beta_bar, float* w3_bar, int64_t ldb,
+                                          void* ws, uint64_t ws_bytes, void* stream) {
+  int st = check_dims(S, 0, n);
+  if (st != COH_OK) return st;
+  if (S == 0) return COH_OK;
+  COH_REQUIRE(w3 && norm && keep && q_bar_next && beta_bar && w3_bar && ws, COH_ERR_VALUE,
+              "NULL pointer argument");
+  COH_REQUIRE(ldw >= n && ldq >= n && ldb >= n, COH_ERR_VALUE, "inconsistent strides");
+  uint64_t need = 0;
+  coh_probe_step_workspace_bytes(S, n, &need);
+  COH_REQUIRE(ws_bytes >= need, COH_ERR_VALUE, "workspace too small");
+  const int nch = (int)std::max<int64_t>(1, ceil_div(n, kChunk));
+  hipStream_t hs = static_cast<hipStream_t>(stream);
+  double* part = static_cast<double*>(ws);
+  hipLaunchKernelGGL(k_lz_adj_head_part, dim3(nch, (unsigned)S), dim3(256), 0, hs, n, w3, ldw,
+                     norm, q_bar_next, ldq, part, nch);
+  COH_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_lz_adj_head_w, dim3(nch, (unsigned)S), dim3(256), 0, hs, n, w3, ldw, norm,
+                     keep, q_bar_next, ldq, beta_bar, part, nch, w3_bar, ldb);
+  COH_CHECK_LAUNCH();
+  return COH_OK;
+}
+
+extern "C" int coh_probe_step_adjoint_tail(int64_t S, int64_t n, int64_t index, const float* basis,
+                                          int64_t ps, int64_t vs, const float* w0, int64_t ld0,
+                                          const float* w1_bar, int64_t ld1, const float* alpha,
+                                          const uint8_t* active, const float* alpha_bar,
+                                          const float* prev_beta, float* basis_bar, int64_t bps,
+                                          int64_t bvs, float* w0_bar, int64_t ldo,
+                                          float* prev_beta_bar, void* ws, uint64_t ws_bytes,
+                                          void* stream) {
+  int st = check_dims(S, 0, n);
+  if (st != COH_OK) return st;
+  if (S == 0) return COH_OK;
+  COH_REQUIRE(basis && w0 && w1_bar && alpha && active && alpha_bar && prev_beta && basis_bar &&
+                  w0_bar && prev_beta_bar && ws,
+              COH_ERR_VALUE, "NULL pointer argument");
+  COH_REQUIRE(index >= 0 && vs >= n && ps >= (index + 1) * vs && bvs >= n &&
+                  bps >= (index + 1) * bvs && ld0 >= n && ld1 >= n && ldo >= n,
+              COH_ERR_VALUE, "inconsistent strides");
+  uint64_t need = 0;
+  coh_probe_step_workspace_bytes(S, n, &need);
+  COH_REQUIRE(ws_bytes >= need, COH_ERR_VALUE, "workspace too small");
+  const int nch = (int)std::max<int64_t>(1, ceil_div(n, kChunk));
+  hipStream_t hs = static_cast<hipStream_t>(stream);
+  double* part = static_cast<double*>(ws);
+  double* part2 = part + (int64_t)S * nch;
+  hipLaunchKernelGGL(k_lz_adj_tail_part, dim3(nch, (unsigned)S), dim3(256), 0, hs, n,
+                     basis + index * vs, ps, w1_bar, ld1, part, nch);
+  COH_CHECK_LAUNCH();
+  AdjTailArgs a{n, index, ps, vs, ld0, ld1, bps, bvs, ldo, basis, w0, w1_bar, alpha, active,
+                alpha_bar, prev_beta, basis_bar, w0_bar, part, part2, nch};
+  hipLaunchKernelGGL(k_lz_adj_tail_w, dim3(nch, (unsigned)S), dim3(256), 0, hs, a);
+  COH_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_lz_adj_tail_final, dim3((unsigned)ceil_div(S, 64)), dim3(64), 0, hs, S,
+                     nch, part2, prev_beta_bar);
+  COH_CHECK_LAUNCH();
   return COH_OK;
 }

@@ -122,6 +122,17 @@ def _rows(t):
     return t.stride(0) if t.shape[0] > 1 else t.shape[-1]
 
 
+def _step_workspace(dev, S, n):
+    nbytes = ctypes.c_uint64()
+    N.call("coh_probe_step_workspace_bytes", S, n, ctypes.byref(nbytes))
+    key = (dev, int(nbytes.value))
+    ws = getattr(_STEP_LOCAL, "ws", {}).get(key)
+    if ws is None:
+        ws = torch.empty(max(int(nbytes.value), 256), dtype=torch.uint8, device=dev)
+        _STEP_LOCAL.ws = {key: ws}
+    return ws, nbytes
+
+
 def _step_parts_device(basis, product, previous_beta, index, next_out=None):
     """The quantities of :func:`_step_parts` from the fused step kernels
     (coh_probe_step_prep, the two re-orthogonalisation passes,
@@ -136,13 +147,7 @@ def _step_parts_device(basis, product, previous_beta, index, next_out=None):
     if product.stride(1) != 1:
         product = product.contiguous()
     pb = previous_beta.to(device=dev, dtype=torch.float32).contiguous()
-    nbytes = ctypes.c_uint64()
-    N.call("coh_probe_step_workspace_bytes", S, n, ctypes.byref(nbytes))
-    key = (dev, int(nbytes.value))
-    ws = getattr(_STEP_LOCAL, "ws", {}).get(key)
-    if ws is None:
-        ws = torch.empty(max(int(nbytes.value), 256), dtype=torch.uint8, device=dev)
-        _STEP_LOCAL.ws = {key: ws}
+    ws, nbytes = _step_workspace(dev, S, n)
     w0 = torch.empty((S, n), dtype=torch.float32, device=dev)
     w1 = torch.empty_like(w0)
     alpha = torch.empty(S, dtype=torch.float32, device=dev)
@@ -220,6 +225,10 @@ def _step_pullback(basis, product, previous_beta, index, q_bar_next, alpha_bar, 
                    basis_bar):
     """Adjoint of :func:`_step`; adds the basis cotangent into ``basis_bar``
     (rows 0..index) and returns (product_bar, previous_beta_bar)."""
+    if (_on_device(basis, product) and basis.stride(2) == 1 and basis_bar.stride(2) == 1
+            and q_bar_next.stride(-1) == 1):
+        return _step_pullback_device(basis, product, previous_beta, index, q_bar_next,
+                                     alpha_bar, beta_bar, basis_bar)
     s = _step_parts(basis, product, previous_beta, index)
     keep = s["keep"][:, None]
     norm = s["norm"][:, None]
@@ -260,6 +269,53 @@ def _step_pullback(basis, product, previous_beta, index, q_bar_next, alpha_bar, 
     if index > 0:
         basis_bar[:, index - 1, :] += -previous_beta[:, None] * w0b
     return w0b, beta_prev_bar
+
+
+def _step_pullback_device(basis, product, previous_beta, index, q_bar_next, alpha_bar, beta_bar,
+                          basis_bar):
+    """:func:`_step_pullback` on the device: the recomputed step, then
+    coh_probe_step_adjoint_head, the re-orthogonalisation adjoint (two
+    dots/project-out passes and one rank-4 update of the basis cotangent) and
+    coh_probe_step_adjoint_tail."""
+    s = _step_parts_device(basis, product, previous_beta, index)
+    S, _, n = basis.shape
+    dev = basis.device
+    f32 = dict(device=dev, dtype=torch.float32)
+    ws, nbytes = _step_workspace(dev, S, n)
+    w3 = s["w3"]
+    bbar = beta_bar.to(**f32).contiguous()
+    w3b = torch.empty((S, n), **f32)
+    N.call("coh_probe_step_adjoint_head", S, n, N.ptr(w3), _rows(w3), N.ptr(s["norm"]),
+           N.ptr(s["keep"]), N.ptr(q_bar_next), _rows(q_bar_next), N.ptr(bbar), N.ptr(w3b), n,
+           N.ptr(ws), int(nbytes.value), N.stream_ptr())
+    P = s["P"]
+    # pass 2: w3 = w2 - P c2, c2 = P^T w2;  pass 1 likewise
+    d2 = _probe_dots(P, w3b)
+    w2b = _project_out(P, d2, w3b)
+    d1 = _probe_dots(P, w2b)
+    w1b = _project_out(P, d1, w2b)
+    # basis_bar[j] -= c2[j] w3b + d2[j] w2 + c1[j] w2b + d1[j] w1
+    coeffs = torch.stack([s["c2"], d2, s["c1"], d1], dim=2).neg_()        # (S, r, 4)
+    vecs = torch.stack([w3b, s["w2"], w2b, s["w1"]], dim=1)                # (S, 4, n)
+    bb = basis_bar[:, : index + 1, :]
+    r = P.shape[1]
+    N.call("coh_probe_rank_update", S, r, n, 4, N.ptr(coeffs), coeffs.stride(0),
+           coeffs.stride(1), N.ptr(vecs), vecs.stride(0), vecs.stride(1), N.ptr(bb),
+           basis_bar.stride(0) if S > 1 else basis_bar.shape[1] * basis_bar.stride(1),
+           basis_bar.stride(1), N.stream_ptr())
+    abar = alpha_bar.to(**f32).contiguous()
+    pb = previous_beta.to(**f32).contiguous()
+    w0, w1b = s["w0"], w1b.contiguous()
+    w0b = torch.empty((S, n), **f32)
+    pbb = torch.empty(S, **f32)
+    N.call("coh_probe_step_adjoint_tail", S, n, int(index), N.ptr(basis),
+           basis.stride(0) if S > 1 else basis.shape[1] * basis.stride(1), basis.stride(1),
+           N.ptr(w0), _rows(w0), N.ptr(w1b), _rows(w1b), N.ptr(s["alpha"]), N.ptr(s["active"]),
+           N.ptr(abar), N.ptr(pb), N.ptr(basis_bar),
+           basis_bar.stride(0) if S > 1 else basis_bar.shape[1] * basis_bar.stride(1),
+           basis_bar.stride(1), N.ptr(w0b), n, N.ptr(pbb), N.ptr(ws), int(nbytes.value),
+           N.stream_ptr())
+    return w0b, pbb
 
 
 def quadrature(alphas, betas, weight):

@@ -304,6 +304,31 @@ int coh_probe_step_finish(int64_t S, int64_t n, const float* dev_w3, int64_t ldw
                           float* dev_beta, float* dev_alpha_out, float* dev_next, int64_t ldn,
                           void* dev_workspace, uint64_t workspace_bytes, void* stream);
 
+/* Reverse of the step above (the VJP of slq.py:49-74, taken by the reverse
+ * sweep of slq.py:124-140), around the re-orthogonalisation adjoint
+ * (coh_probe_dots / coh_probe_project_out / coh_probe_rank_update).
+ * Head: with nxt = w3 / norm and proj = nxt . q_bar_next,
+ *   w3_bar = keep ? (q_bar_next - nxt * proj) / norm + beta_bar * nxt : 0.
+ * Tail: with a_tot = (active ? alpha_bar : 0) - q . w1_bar,
+ *   w0_bar = w1_bar + q * a_tot (the product's cotangent),
+ *   basis_bar[index] += -alpha * w1_bar + a_tot * w0,
+ *   basis_bar[index - 1] += -prev_beta * w0_bar (index > 0),
+ *   prev_beta_bar = -(prev . w0_bar).
+ * Workspace: coh_probe_step_workspace_bytes. */
+int coh_probe_step_adjoint_head(int64_t S, int64_t n, const float* dev_w3, int64_t ldw,
+                                const float* dev_norm, const uint8_t* dev_keep,
+                                const float* dev_q_bar_next, int64_t ldq,
+                                const float* dev_beta_bar, float* dev_w3_bar, int64_t ldb,
+                                void* dev_workspace, uint64_t workspace_bytes, void* stream);
+int coh_probe_step_adjoint_tail(int64_t S, int64_t n, int64_t index, const float* dev_basis,
+                                int64_t ps, int64_t vs, const float* dev_w0, int64_t ld0,
+                                const float* dev_w1_bar, int64_t ld1, const float* dev_alpha,
+                                const uint8_t* dev_active, const float* dev_alpha_bar,
+                                const float* dev_prev_beta, float* dev_basis_bar, int64_t bps,
+                                int64_t bvs, float* dev_w0_bar, int64_t ldo,
+                                float* dev_prev_beta_bar, void* dev_workspace,
+                                uint64_t workspace_bytes, void* stream);
+
 /* ---------------------------------------------------------------------
  * Kernel timing instrumentation (the reference only logs wall-clock
  * per-iteration fields, reml.py:1979-1980).  When enabled, each main

@@ -75,3 +75,32 @@ def _compare(slq, basis, product, beta, index):
     torch.testing.assert_close(nxt.cpu(), nxt_ref, rtol=1e-4, atol=2e-5)
     assert float(b[1]) == 0.0 and float(nxt[1].abs().max()) == 0.0
     assert float(a[0]) == 1.0
+
+
+@pytest.mark.parametrize("n,index,S", [(3000, 0, 5), (4096 + 17, 1, 5), (2048 * 3, 4, 5),
+                                       (700, 6, 5), (2500, 3, 1)])
+def test_fused_step_adjoint_matches_elementwise_adjoint(n, index, S):
+    """coh_probe_step_adjoint_head/tail with the re-orthogonalisation adjoint
+    against the element-wise torch adjoint of the same step (run on the CPU
+    in FP32), including the inactive and breakdown probes."""
+    from coherit_amd import slq
+
+    basis, product, beta = _inputs(max(S, 3), n, index, seed=7 * n + index)
+    if S == 1:
+        basis, product, beta = basis[2:].contiguous(), product[2:].contiguous(), beta[2:]
+    rng = np.random.default_rng(index)
+    depth1 = basis.shape[1]
+    q_bar = torch.as_tensor(rng.standard_normal((S, depth1, n)).astype(np.float32), device="cuda")
+    a_bar = torch.as_tensor(rng.standard_normal(S).astype(np.float32), device="cuda")
+    b_bar = torch.as_tensor(rng.standard_normal(S).astype(np.float32), device="cuda")
+    qb_ref = q_bar.cpu().clone()
+    dw_ref, db_ref = slq._step_pullback(basis.cpu(), product.cpu(), beta.cpu(), index,
+                                        qb_ref[:, index + 1, :], a_bar.cpu(), b_bar.cpu(), qb_ref)
+    dw, db = slq._step_pullback(basis, product, beta, index, q_bar[:, index + 1, :], a_bar,
+                                b_bar, q_bar)
+    scale = float(dw_ref.abs().max())
+    torch.testing.assert_close(dw.cpu(), dw_ref, rtol=1e-4, atol=2e-5 * scale)
+    if index > 0:
+        torch.testing.assert_close(db.cpu(), db_ref, rtol=1e-4, atol=1e-4)
+    scale = float(qb_ref.abs().max())
+    torch.testing.assert_close(q_bar.cpu(), qb_ref, rtol=1e-4, atol=2e-5 * scale)
