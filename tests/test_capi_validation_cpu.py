@@ -1,0 +1,125 @@
+"""CPU: argument checks of the fused Lanczos-step entry points.
+
+Every call here fails its host-side checks (or has nothing to do) before any
+launch, so no GPU is needed; the error codes and messages are the ones the
+Python layer turns into ValueError."""
+import ctypes
+import pathlib
+
+import pytest
+
+ROOT = pathlib.Path(__file__).resolve().parents[1]
+LIB = ROOT / "coherit_amd" / "_lib" / "libcoherit_amd.so"
+COH_OK, COH_ERR_VALUE = 0, 1
+
+pytestmark = pytest.mark.skipif(not LIB.exists(), reason="HIP library not built")
+
+
+@pytest.fixture(scope="module")
+def dll():
+    from coherit_amd import _native
+
+    return _native.load_library(LIB)
+
+
+@pytest.fixture
+def buf():
+    """A host buffer standing in for every pointer argument (never read)."""
+    b = ctypes.create_string_buffer(4096)
+    return ctypes.cast(b, ctypes.c_void_p).value, b
+
+
+def _ws_bytes(dll, S, n):
+    out = ctypes.c_uint64()
+    assert dll.coh_probe_step_workspace_bytes(S, n, ctypes.byref(out)) == COH_OK
+    return out.value
+
+
+@pytest.mark.parametrize("S,n,expect", [(5, 3000, 256), (16, 100_000, 18_944), (1, 1, 256),
+                                        (0, 10, 256)])
+def test_step_workspace_holds_three_partials_per_probe_and_chunk(dll, S, n, expect):
+    # chunks of 2048 elements; S * chunks * 3 doubles, rounded up to 256 bytes
+    assert _ws_bytes(dll, S, n) == expect
+
+
+def _prep(dll, p, S=4, n=3000, index=2, ps=None, vs=None, ldp=None, ldw=None, ws_bytes=None,
+          null_basis=False):
+    vs = n if vs is None else vs
+    ps = (index + 2) * vs if ps is None else ps
+    ldp = n if ldp is None else ldp
+    ldw = n if ldw is None else ldw
+    ws_bytes = _ws_bytes(dll, S, n) if ws_bytes is None else ws_bytes
+    basis = None if null_basis else p
+    return dll.coh_probe_step_prep(S, n, index, basis, ps, vs, p, ldp, p, p, p, ldw, p, p, p, p,
+                                   ws_bytes, None)
+
+
+@pytest.mark.parametrize("kw,msg", [
+    (dict(ldp=2999), b"inconsistent strides"),
+    (dict(ldw=100), b"inconsistent strides"),
+    (dict(vs=2000), b"inconsistent strides"),
+    (dict(ps=3000 * 2), b"inconsistent strides"),       # rows 0..index need 3 vectors
+    (dict(index=-1), b"inconsistent strides"),
+    (dict(null_basis=True), b"NULL pointer argument"),
+    (dict(ws_bytes=8), b"workspace too small"),
+    (dict(n=-5, ws_bytes=1 << 20), b"negative dimension"),
+    (dict(S=70_000, ws_bytes=1 << 30), b"at most 65535 probes per call"),
+])
+def test_step_prep_rejects_bad_arguments(dll, buf, kw, msg):
+    assert _prep(dll, buf[0], **kw) == COH_ERR_VALUE
+    assert msg in dll.coh_last_error()
+
+
+def test_step_calls_with_no_probes_do_nothing(dll, buf):
+    p = buf[0]
+    assert _prep(dll, p, S=0) == COH_OK
+    assert dll.coh_probe_step_finish(0, 100, None, 100, None, None, None, None, None, None, None,
+                                     None, 100, None, 0, None) == COH_OK
+    assert dll.coh_probe_step_adjoint_head(0, 100, None, 100, None, None, None, 100, None, None,
+                                           100, None, 0, None) == COH_OK
+    assert dll.coh_probe_step_adjoint_tail(0, 100, 0, None, 200, 100, None, 100, None, 100, None,
+                                           None, None, None, None, 200, 100, None, 100, None,
+                                           None, 0, None) == COH_OK
+
+
+def test_step_finish_rejects_bad_arguments(dll, buf):
+    p, n, S = buf[0], 3000, 4
+    ws = _ws_bytes(dll, S, n)
+    # next vector with a row stride below n
+    assert dll.coh_probe_step_finish(S, n, p, n, p, p, p, p, p, p, p, p, n - 1, p, ws,
+                                     None) == COH_ERR_VALUE
+    assert b"inconsistent strides" in dll.coh_last_error()
+    # missing norm output
+    assert dll.coh_probe_step_finish(S, n, p, n, p, p, p, None, p, None, None, None, n, p, ws,
+                                     None) == COH_ERR_VALUE
+    assert b"NULL pointer argument" in dll.coh_last_error()
+    assert dll.coh_probe_step_finish(S, n, p, n, p, p, p, p, p, None, None, None, n, p, ws - 8,
+                                     None) == COH_ERR_VALUE
+    assert b"workspace too small" in dll.coh_last_error()
+
+
+def test_step_adjoint_rejects_bad_arguments(dll, buf):
+    p, n, S, index = buf[0], 3000, 4, 3
+    ws = _ws_bytes(dll, S, n)
+    # head: q_bar_next row stride below n
+    assert dll.coh_probe_step_adjoint_head(S, n, p, n, p, p, p, n - 1, p, p, n, p, ws,
+                                           None) == COH_ERR_VALUE
+    assert b"inconsistent strides" in dll.coh_last_error()
+    assert dll.coh_probe_step_adjoint_head(S, n, p, n, p, p, None, n, p, p, n, p, ws,
+                                           None) == COH_ERR_VALUE
+    assert b"NULL pointer argument" in dll.coh_last_error()
+
+    def tail(bps=(index + 2) * n, bvs=n, ps=(index + 2) * n, ld1=n, ws_bytes=ws, pbb=p):
+        return dll.coh_probe_step_adjoint_tail(S, n, index, p, ps, n, p, n, p, ld1, p, p, p, p,
+                                               p, bps, bvs, p, n, pbb, p, ws_bytes, None)
+
+    # the basis cotangent must hold rows 0..index of every probe
+    assert tail(bps=index * n) == COH_ERR_VALUE
+    assert b"inconsistent strides" in dll.coh_last_error()
+    assert tail(bvs=n - 1) == COH_ERR_VALUE
+    assert tail(ps=index * n) == COH_ERR_VALUE
+    assert tail(ld1=10) == COH_ERR_VALUE
+    assert tail(pbb=None) == COH_ERR_VALUE
+    assert b"NULL pointer argument" in dll.coh_last_error()
+    assert tail(ws_bytes=0) == COH_ERR_VALUE
+    assert b"workspace too small" in dll.coh_last_error()
