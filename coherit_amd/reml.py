@@ -364,15 +364,18 @@ def _multi_slq_logdet_and_score(ctx, Hv_fn, pvec, key, *, nsamples, m, use_refer
         left, right = whiten(left), whiten(right)
         result = torch.zeros_like(left)
         scores = []
+        right64 = right.double()
         gen = _apply_genetic_stack(ctx, left)
         for k in range(ctx.G):
             result = result + float(pvec[k]) * gen[k]
-            scores.append(float((right.double() * gen[k].double()).sum()))
+            scores.append((right64 * gen[k].double()).sum())
         for e in range(ctx.E):
             prod = left if ctx.residual_diag_stack is None else ctx.residual_diag_stack[e, :, None] * left
             result = result + float(pvec[ctx.G + e]) * prod
-            scores.append(float((right.double() * prod.double()).sum()))
-        return whiten(result), np.asarray(scores)
+            scores.append((right64 * prod.double()).sum())
+        # FP64 scores stay on the device (no host wait per Lanczos step);
+        # the reverse sweep sums them there
+        return whiten(result), torch.stack(scores)
 
     ref_logdet = projected_core_logdet(reference, ctx.n) if reference is not None else 0.0
     value, gradient = _slq_value_and_grad(

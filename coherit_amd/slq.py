@@ -367,10 +367,17 @@ def _batch_pullback(matvec_pullback, basis, products, beta, a_bar, b_bar):
                                 q_bar[:, index + 1, :], a_bar_t[index], b_bar_t[index], q_bar)
         input_bar, part = matvec_pullback(dw.T.contiguous(), basis[:, index, :].T.contiguous())
         q_bar[:, index, :] += torch.as_tensor(input_bar).T
-        part = np.asarray(part, dtype=np.float64)
+        if torch.is_tensor(part):
+            # a device pullback's scores stay on the device: the sum runs
+            # there in the same order, and the host waits once per batch
+            part = part.to(torch.float64)
+        else:
+            part = np.asarray(part, dtype=np.float64)
         gradient = part if gradient is None else gradient + part
         if index:
             b_bar_t[index - 1] += db
+    if torch.is_tensor(gradient):
+        gradient = gradient.cpu().numpy()
     return gradient
 
 
@@ -381,7 +388,7 @@ def logdet_value_and_grad(matvec, matvec_pullback, n, key, *, nsamples, m,
 
     ``key`` is the integer probe seed.  ``matvec(V (n, k)) -> (n, k)``;
     ``matvec_pullback(left, right) -> (matvec(left), gradient)`` with
-    gradient[k] = sum(right * D_k(left)).  ``accept_value(value)`` gates the
+    gradient[k] = sum(right * D_k(left)) (a NumPy array or a device tensor).  ``accept_value(value)`` gates the
     (deferred, when it fits the workspace) reverse pass.  Returns
     ``(value, gradient or None)``; value is a float, gradient a NumPy array.
     """
