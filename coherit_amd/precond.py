@@ -68,6 +68,7 @@ class ProjectedCoreRuntime:
     chol: torch.Tensor
     eigvals: Optional[torch.Tensor] = None
     eigvecs: Optional[torch.Tensor] = None
+    eig_sqrt: Optional[torch.Tensor] = None   # sqrt(max(eigvals, 1e-8)), kept per runtime
 
 
 def _apply_scalar_diag(v, d_inv, chol, U):
@@ -82,13 +83,15 @@ def _apply_scalar_diag(v, d_inv, chol, U):
     return out[:, 0] if vec else out
 
 
-def _apply_scalar_diag_invsqrt(v, d_inv_sqrt, eigvals, eigvecs, U):
+def _apply_scalar_diag_invsqrt(v, d_inv_sqrt, eigvals, eigvecs, U, eig_sqrt=None):
     """precond.py:61-84: M^{-1/2} v via the eigenpairs of d I + C."""
     vec = v.ndim == 1
     V = v[:, None] if vec else v
     rhs = tall_t_matmul(U, V).double()
     coeff = eigvecs.T @ rhs
-    scaled = coeff / torch.sqrt(torch.clamp(eigvals, min=1e-8))[:, None]
+    if eig_sqrt is None:
+        eig_sqrt = torch.sqrt(torch.clamp(eigvals, min=1e-8))
+    scaled = coeff / eig_sqrt[:, None]
     z = eigvecs @ scaled
     out = d_inv_sqrt * V + U @ (z - d_inv_sqrt * rhs).to(torch.float32)
     return out[:, 0] if vec else out
@@ -156,12 +159,13 @@ def build_projected_core_runtime(precond_conf, theta_g, diag_H, eps: float = 0.0
     chol, info = torch.linalg.cholesky_ex(middle)
     if int(info) != 0:
         raise FloatingPointError("Projected-core middle matrix is not positive definite.")
-    eigvals = eigvecs = None
+    eigvals = eigvecs = eig_sqrt = None
     if need_invsqrt:
         ev, eigvecs = torch.linalg.eigh(middle)
         eigvals = torch.clamp(ev, min=1e-8)
+        eig_sqrt = torch.sqrt(torch.clamp(eigvals, min=1e-8))
     return ProjectedCoreRuntime(U=U, total_rank=r, d=d, d_inv=1.0 / d, chol=chol,
-                                eigvals=eigvals, eigvecs=eigvecs)
+                                eigvals=eigvals, eigvecs=eigvecs, eig_sqrt=eig_sqrt)
 
 
 def make_projected_core_precond_from_runtime(runtime) -> Optional[Callable]:
@@ -182,7 +186,7 @@ def projected_core_apply_invsqrt(runtime, v):
             "Projected-core runtime missing eigendecomposition required for inverse-square-root apply."
         )
     return _apply_scalar_diag_invsqrt(v, float(np.sqrt(runtime.d_inv)), runtime.eigvals,
-                                      runtime.eigvecs, runtime.U)
+                                      runtime.eigvecs, runtime.U, runtime.eig_sqrt)
 
 
 def projected_core_logdet(runtime, n_dim: int) -> float:
