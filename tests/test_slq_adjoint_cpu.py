@@ -101,3 +101,43 @@ def test_full_depth_value_matches_dense_probe_oracle():
     coeff = vecs.T @ probes.T
     gm = vecs @ (divided * (coeff @ coeff.T / 4)) @ vecs.T
     np.testing.assert_allclose(grad, np.einsum("ij,kij->k", gm, directions), rtol=1e-5, atol=1e-6)
+
+
+def test_pullback_scores_as_tensors_sum_to_the_same_gradient():
+    """A pullback may return its scores as a tensor (a device pullback keeps
+    them on the device); the reverse sweep then sums them as tensors, in the
+    same order, to the same FP64 gradient as from NumPy scores."""
+    directions, theta = problem("regular")
+    n, depth = directions.shape[1], 12
+    probes = slq.rademacher_probes(73, n, 4).astype(np.float64)
+    D = torch.as_tensor(directions)
+    H = torch.as_tensor(np.einsum("k,kij->ij", theta, directions))
+    S = probes.shape[0]
+    basis = torch.zeros((S, depth + 1, n), dtype=torch.float64)
+    basis[:, 0] = torch.as_tensor(probes) / np.sqrt(n)
+    products = torch.zeros((S, depth, n), dtype=torch.float64)
+    alphas, betas, prev = [], [], torch.zeros(S, dtype=torch.float64)
+    for i in range(depth):
+        prod = (H @ basis[:, i].T).T
+        nq, a, b = slq._step(basis, prod, prev, i)
+        products[:, i], basis[:, i + 1] = prod, nq
+        alphas.append(a)
+        betas.append(b)
+        prev = b
+    A = torch.stack(alphas).numpy()
+    B = torch.stack(betas[:-1]).numpy()
+    _, a_bar, b_small = slq.quadrature(A, B, n / S)
+    b_bar = np.concatenate([b_small, np.zeros((1, S))])
+
+    def pull(as_tensor):
+        def pullback(left, right):
+            prods = torch.einsum("kij,js->kis", D, left)
+            scores = torch.stack([(right * prods[k]).sum() for k in range(len(theta))])
+            out = torch.einsum("k,kis->is", torch.as_tensor(theta), prods)
+            return out, (scores if as_tensor else scores.numpy())
+        return pullback
+
+    g_np = slq._batch_pullback(pull(False), basis, products, torch.stack(betas), a_bar, b_bar)
+    g_t = slq._batch_pullback(pull(True), basis, products, torch.stack(betas), a_bar, b_bar)
+    assert isinstance(g_t, np.ndarray) and g_t.dtype == np.float64
+    np.testing.assert_array_equal(g_t, g_np)
