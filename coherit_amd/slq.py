@@ -122,6 +122,12 @@ def _rows(t):
     return t.stride(0) if t.shape[0] > 1 else t.shape[-1]
 
 
+def _probes(t):
+    """Probe stride of a probe-major (S, rows, n) tensor with unit element
+    stride; with one probe only its own rows are read, whatever stride(0)."""
+    return t.stride(0) if t.shape[0] > 1 else t.shape[1] * t.stride(1)
+
+
 def _step_workspace(dev, S, n):
     nbytes = ctypes.c_uint64()
     N.call("coh_probe_step_workspace_bytes", S, n, ctypes.byref(nbytes))
@@ -154,7 +160,7 @@ def _step_parts_device(basis, product, previous_beta, index, next_out=None):
     active = torch.empty(S, dtype=torch.bool, device=dev)
     scale = torch.empty(S, dtype=torch.float32, device=dev)
     N.call("coh_probe_step_prep", S, n, int(index), N.ptr(basis),
-           basis.stride(0) if S > 1 else basis.shape[1] * basis.stride(1), basis.stride(1),
+           _probes(basis), basis.stride(1),
            N.ptr(product), _rows(product), N.ptr(pb), N.ptr(w0), N.ptr(w1), _rows(w0),
            N.ptr(alpha), N.ptr(active), N.ptr(scale), N.ptr(ws), int(nbytes.value),
            N.stream_ptr())
@@ -301,7 +307,7 @@ def _step_pullback_device(basis, product, previous_beta, index, q_bar_next, alph
     r = P.shape[1]
     N.call("coh_probe_rank_update", S, r, n, 4, N.ptr(coeffs), coeffs.stride(0),
            coeffs.stride(1), N.ptr(vecs), vecs.stride(0), vecs.stride(1), N.ptr(bb),
-           basis_bar.stride(0) if S > 1 else basis_bar.shape[1] * basis_bar.stride(1),
+           _probes(basis_bar),
            basis_bar.stride(1), N.stream_ptr())
     abar = alpha_bar.to(**f32).contiguous()
     pb = previous_beta.to(**f32).contiguous()
@@ -309,10 +315,10 @@ def _step_pullback_device(basis, product, previous_beta, index, q_bar_next, alph
     w0b = torch.empty((S, n), **f32)
     pbb = torch.empty(S, **f32)
     N.call("coh_probe_step_adjoint_tail", S, n, int(index), N.ptr(basis),
-           basis.stride(0) if S > 1 else basis.shape[1] * basis.stride(1), basis.stride(1),
+           _probes(basis), basis.stride(1),
            N.ptr(w0), _rows(w0), N.ptr(w1b), _rows(w1b), N.ptr(s["alpha"]), N.ptr(s["active"]),
            N.ptr(abar), N.ptr(pb), N.ptr(basis_bar),
-           basis_bar.stride(0) if S > 1 else basis_bar.shape[1] * basis_bar.stride(1),
+           _probes(basis_bar),
            basis_bar.stride(1), N.ptr(w0b), n, N.ptr(pbb), N.ptr(ws), int(nbytes.value),
            N.stream_ptr())
     return w0b, pbb
@@ -388,8 +394,9 @@ def logdet_value_and_grad(matvec, matvec_pullback, n, key, *, nsamples, m,
 
     ``key`` is the integer probe seed.  ``matvec(V (n, k)) -> (n, k)``;
     ``matvec_pullback(left, right) -> (matvec(left), gradient)`` with
-    gradient[k] = sum(right * D_k(left)) (a NumPy array or a device tensor).  ``accept_value(value)`` gates the
-    (deferred, when it fits the workspace) reverse pass.  Returns
+    gradient[k] = sum(right * D_k(left)), a NumPy array or a device tensor.
+    ``accept_value(value)`` gates the (deferred, when it fits the workspace)
+    reverse pass.  Returns
     ``(value, gradient or None)``; value is a float, gradient a NumPy array.
     """
     depth = min(int(m), int(n))
