@@ -174,6 +174,31 @@ __global__ void k_safe_div(int64_t n, int64_t k, float* __restrict__ x,
   }
 }
 
+// Tail of the projected-core preconditioner applications (precond.py:42-58,
+// 61-84), as the reference's element-wise expressions without contraction:
+//   out = (float)(z - d * rhs)   over an (r, k) block, any strides
+//   y   = (float)d * x + g       over a contiguous FP32 block
+__global__ void k_core_correction(int64_t r, int64_t k, double d, const double* __restrict__ z,
+                                  int64_t zs0, int64_t zs1, const double* __restrict__ rhs,
+                                  int64_t rs0, int64_t rs1, float* __restrict__ out, int64_t os0,
+                                  int64_t os1) {
+#pragma clang fp contract(off)
+  const int64_t total = r * k;
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = idx / k, j = idx % k;
+    out[i * os0 + j * os1] = static_cast<float>(z[i * zs0 + j * zs1] - d * rhs[i * rs0 + j * rs1]);
+  }
+}
+
+__global__ void k_scale_add(int64_t total, float d, const float* __restrict__ x,
+                            const float* __restrict__ g, float* __restrict__ y) {
+#pragma clang fp contract(off)
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x)
+    y[idx] = d * x[idx] + g[idx];
+}
+
 static unsigned ew_grid(int64_t total) {
   return (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(total, 256), 16384));
 }
@@ -280,6 +305,33 @@ extern "C" int coh_lanczos_step(int64_t n, int64_t k, const float* v, const floa
   hipLaunchKernelGGL(k_sqrt_vec, dim3((unsigned)ceil_div(k, 256)), dim3(256), 0, st, k, ww, beta);
   COH_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_safe_div, dim3(ew_grid(n * k)), dim3(256), 0, st, n, k, v_next, beta);
+  COH_CHECK_LAUNCH();
+  return COH_OK;
+}
+
+extern "C" int coh_core_correction(int64_t r, int64_t k, double d, const double* z, int64_t zs0,
+                                   int64_t zs1, const double* rhs, int64_t rs0, int64_t rs1,
+                                   float* out, int64_t os0, int64_t os1, void* stream) {
+  COH_REQUIRE(r >= 0 && k >= 0, COH_ERR_VALUE, "invalid shape");
+  if (r == 0 || k == 0) return COH_OK;
+  COH_REQUIRE(z && rhs && out, COH_ERR_VALUE, "NULL argument");
+  COH_REQUIRE(zs0 >= 0 && zs1 >= 0 && rs0 >= 0 && rs1 >= 0 && os0 >= 0 && os1 >= 0,
+              COH_ERR_VALUE, "negative stride");
+  COH_REQUIRE((os1 == 1 && (r == 1 || os0 >= k)) || (os0 == 1 && (k == 1 || os1 >= r)),
+              COH_ERR_VALUE, "output must be row- or column-major without overlap");
+  hipLaunchKernelGGL(k_core_correction, dim3(ew_grid(r * k)), dim3(256), 0, (hipStream_t)stream,
+                     r, k, d, z, zs0, zs1, rhs, rs0, rs1, out, os0, os1);
+  COH_CHECK_LAUNCH();
+  return COH_OK;
+}
+
+extern "C" int coh_scale_add(int64_t count, double d, const float* x, const float* g, float* y,
+                             void* stream) {
+  COH_REQUIRE(count >= 0, COH_ERR_VALUE, "invalid shape");
+  if (count == 0) return COH_OK;
+  COH_REQUIRE(x && g && y, COH_ERR_VALUE, "NULL argument");
+  hipLaunchKernelGGL(k_scale_add, dim3(ew_grid(count)), dim3(256), 0, (hipStream_t)stream, count,
+                     static_cast<float>(d), x, g, y);
   COH_CHECK_LAUNCH();
   return COH_OK;
 }

@@ -12,6 +12,8 @@ from typing import Callable, Optional
 import numpy as np
 import torch
 
+from . import _native as N
+
 
 def tall_t_matmul(A: torch.Tensor, B: torch.Tensor, chunks: int = 50) -> torch.Tensor:
     """``A^T B`` for tall ``A`` (n, r) and ``B`` (n, k) of one dtype (FP32 or FP64).
@@ -71,6 +73,39 @@ class ProjectedCoreRuntime:
     eig_sqrt: Optional[torch.Tensor] = None   # sqrt(max(eigvals, 1e-8)), kept per runtime
 
 
+def _core_correction32(z, rhs, d):
+    """(z - d * rhs) rounded to FP32, for FP64 (r, k) blocks; one launch of
+    coh_core_correction on the device (same operations, same rounding).
+    The result takes z's layout when z is column-major (as a Cholesky solve
+    returns it), as the torch expression's does, so the product that
+    follows runs the same GEMM."""
+    if z.is_cuda and z.dtype == torch.float64 and rhs.dtype == torch.float64 \
+            and z.dim() == 2 and z.shape == rhs.shape:
+        r, k = z.shape
+        if r > 1 and k > 1 and z.stride(0) == 1 and z.stride(1) == r:
+            out = torch.empty((k, r), dtype=torch.float32, device=z.device).T
+        else:
+            out = torch.empty((r, k), dtype=torch.float32, device=z.device)
+        N.call("coh_core_correction", r, k, float(d), N.ptr(z), z.stride(0), z.stride(1),
+               N.ptr(rhs), rhs.stride(0), rhs.stride(1), N.ptr(out), out.stride(0),
+               out.stride(1), N.stream_ptr())
+        return out
+    return (z - d * rhs).to(torch.float32)
+
+
+def _scaled_sum(d, V, G):
+    """d * V + G in FP32; one launch of coh_scale_add on the device for
+    contiguous blocks of one shape (the scalar rounded to FP32 first, as
+    torch does for a float32 tensor)."""
+    if V.is_cuda and V.dtype == torch.float32 and G.dtype == torch.float32 \
+            and V.shape == G.shape and V.is_contiguous() and G.is_contiguous():
+        out = torch.empty_like(G)
+        N.call("coh_scale_add", G.numel(), float(d), N.ptr(V), N.ptr(G), N.ptr(out),
+               N.stream_ptr())
+        return out
+    return d * V + G
+
+
 def _apply_scalar_diag(v, d_inv, chol, U):
     """precond.py:42-58: M^{-1} v = d^{-1} v + U((dI+C)^{-1} - d^{-1}) U^T v."""
     vec = v.ndim == 1
@@ -78,8 +113,8 @@ def _apply_scalar_diag(v, d_inv, chol, U):
     rhs = tall_t_matmul(U, V)                       # (r, k) FP32
     rhs64 = rhs.double()
     z = torch.cholesky_solve(rhs64, chol, upper=False)
-    corr = (z - d_inv * rhs64).to(torch.float32)
-    out = d_inv * V + U @ corr
+    corr = _core_correction32(z, rhs64, d_inv)
+    out = _scaled_sum(d_inv, V, U @ corr)
     return out[:, 0] if vec else out
 
 
@@ -93,7 +128,7 @@ def _apply_scalar_diag_invsqrt(v, d_inv_sqrt, eigvals, eigvecs, U, eig_sqrt=None
         eig_sqrt = torch.sqrt(torch.clamp(eigvals, min=1e-8))
     scaled = coeff / eig_sqrt[:, None]
     z = eigvecs @ scaled
-    out = d_inv_sqrt * V + U @ (z - d_inv_sqrt * rhs).to(torch.float32)
+    out = _scaled_sum(d_inv_sqrt, V, U @ _core_correction32(z, rhs, d_inv_sqrt))
     return out[:, 0] if vec else out
 
 

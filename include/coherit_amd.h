@@ -244,6 +244,19 @@ int coh_pcg_direction_update(int64_t n, int64_t k, float* dev_P, int64_t ldp, co
                              const double* dev_threshold, double* dev_rnorm,
                              int32_t* dev_active, void* dev_workspace,
                              uint64_t workspace_bytes, void* stream);
+/* Tail of the projected-core preconditioner applications M^{-1} v and
+ * M^{-1/2} v (precond.py:42-58, 61-84), the reference's element-wise
+ * expressions in one launch each:
+ *   coh_core_correction: out[i][j] = (float)(z[i][j] - d * rhs[i][j]) over an
+ *     (r, k) block, FP64 in, FP32 out, with element (i, j) of z at
+ *     z + i*zs0 + j*zs1 (rhs and out likewise).
+ *   coh_scale_add: y = (float)d * x + g over `count` contiguous FP32 values. */
+int coh_core_correction(int64_t r, int64_t k, double d, const double* dev_z, int64_t zs0,
+                        int64_t zs1, const double* dev_rhs, int64_t rs0, int64_t rs1,
+                        float* dev_out, int64_t os0, int64_t os1, void* stream);
+int coh_scale_add(int64_t count, double d, const float* dev_x, const float* dev_g, float* dev_y,
+                  void* stream);
+
 /* Affine Lanczos step (reml.py:284-304 `_lanczos_step_jit`):
  * w = Kv - beta_prev*prev; alpha = colsum(v*w); w -= alpha*v;
  * beta = ||w||; v_next = w / (beta > 0 ? beta : 1).  v_next may alias prev. */

@@ -123,3 +123,21 @@ def test_step_adjoint_rejects_bad_arguments(dll, buf):
     assert b"NULL pointer argument" in dll.coh_last_error()
     assert tail(ws_bytes=0) == COH_ERR_VALUE
     assert b"workspace too small" in dll.coh_last_error()
+
+
+def test_preconditioner_tail_entry_points_check_their_arguments(dll, buf):
+    p = buf[0]
+    assert dll.coh_core_correction(4, 3, 0.5, None, 3, 1, p, 3, 1, p, 3, 1, None) == COH_ERR_VALUE
+    assert b"NULL argument" in dll.coh_last_error()
+    assert dll.coh_core_correction(4, 3, 0.5, p, -3, 1, p, 3, 1, p, 3, 1, None) == COH_ERR_VALUE
+    assert b"negative stride" in dll.coh_last_error()
+    assert dll.coh_core_correction(4, 3, 0.5, p, 3, 1, p, 3, 1, p, 0, 1, None) == COH_ERR_VALUE
+    assert b"without overlap" in dll.coh_last_error()
+    assert dll.coh_core_correction(4, 3, 0.5, p, 3, 1, p, 3, 1, p, 2, 1, None) == COH_ERR_VALUE
+    assert dll.coh_core_correction(-1, 3, 0.5, p, 3, 1, p, 3, 1, p, 3, 1, None) == COH_ERR_VALUE
+    assert dll.coh_core_correction(0, 3, 0.5, None, 3, 1, None, 3, 1, None, 3, 1,
+                                   None) == COH_OK
+    assert dll.coh_scale_add(10, 0.5, p, None, p, None) == COH_ERR_VALUE
+    assert b"NULL argument" in dll.coh_last_error()
+    assert dll.coh_scale_add(-2, 0.5, p, p, p, None) == COH_ERR_VALUE
+    assert dll.coh_scale_add(0, 0.5, None, None, None, None) == COH_OK
